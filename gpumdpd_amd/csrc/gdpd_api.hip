@@ -1,0 +1,1353 @@
+// gdpd_api.hip -- host runtime and C ABI of libgdpd (see include/gdpd.h).
+//
+// One gdpd_ctx = one device + one HIP stream (default priority).  Each system
+// ("replica") owns its device arrays; a device array of RepDev descriptors,
+// indexed by system handle, is what the kernels read, so one launch can cover
+// the bricks of many replicas.  Per-step scalars (step counter, buffer parity)
+// live on the device, which makes a captured step graph replayable.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gdpd.h"
+#include "gdpd_internal.h"
+#include "gdpd_kernels.h"
+
+using namespace gdpd;
+
+namespace {
+
+struct Tables {
+    // device work tables for one set of systems
+    WorkItem *beads = nullptr, *cells = nullptr, *scan = nullptr, *bricks_dpd = nullptr,
+             *bricks_mdpd = nullptr;
+    int *rep_chunk_off = nullptr, *rep_list = nullptr, *chunk_sums = nullptr;
+    int n_beads = 0, n_cells = 0, n_scan = 0, n_dpd = 0, n_mdpd = 0, n_reps = 0;
+    BrickLaunch bl_dpd{}, bl_mdpd{};
+    int precision = 0;
+    std::vector<int> key;
+    void release()
+    {
+        void *ps[] = {beads, cells, scan, bricks_dpd, bricks_mdpd, rep_chunk_off, rep_list, chunk_sums};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+        *this = Tables();
+    }
+};
+
+struct Graph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int steps = 0;
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        steps = 0;
+    }
+};
+
+struct Sys {
+    gdpd_system_desc desc{};
+    RepDev h{};                 // host copy of the device descriptor
+    std::vector<void *> allocs; // device allocations owned by this system
+    bool uploaded = false, prepared = false, cells_valid = false, have_pair = false;
+    int style = GDPD_STYLE_NONE;
+    double temp = 1.0, rc = 1.0, rd = 0.75;
+    uint32_t seed = 0;
+    std::vector<double> coeff;  // packed, as given
+    std::vector<double> masses;
+    int tune_b[3] = {0, 0, 0};
+    int resort_every = 0, graph_steps = 0;
+    int steps_since_resort = 0;
+    bool midstep = false;       // between a zero_step/advance and finish
+    Tables tab;                 // single-system work tables
+    Graph graph;                // single-system step graph
+    // reference neighbour list
+    int nl_nmax = 0, nl_nb[3] = {0, 0, 0};
+    int *nl_binof = nullptr, *nl_count = nullptr, *nl_start = nullptr, *nl_fill = nullptr,
+        *nl_content = nullptr, *nl_n = nullptr, *nl_neigh = nullptr, *nl_over = nullptr,
+        *nl_flag = nullptr;
+    double *nl_last = nullptr;
+    bool nl_built = false;
+    int64_t bytes = 0;
+};
+
+} // namespace
+
+struct gdpd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<Sys *> sys;
+    RepDev *d_reps = nullptr;
+    int d_reps_cap = 0;
+    // kernel timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    int64_t force_launches = 0;
+    // batch tables cache
+    Tables batch;
+    Graph batch_graph;
+    int max_lds = 65536;
+};
+
+namespace {
+
+int fail(gdpd_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, GDPD_ERR_HIP, "%s failed: %s (%s:%d)", #call,                    \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                           \
+    } while (0)
+
+template <typename T> int dalloc(gdpd_ctx *ctx, Sys *s, T **p, size_t n)
+{
+    size_t bytes = sizeof(T) * (n > 0 ? n : 1);
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ctx, GDPD_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    if (s) {
+        s->allocs.push_back(*p);
+        s->bytes += (int64_t)bytes;
+    }
+    return GDPD_OK;
+}
+
+Sys *get_sys(gdpd_ctx *ctx, int id)
+{
+    if (!ctx || id < 0 || id >= (int)ctx->sys.size()) return nullptr;
+    return ctx->sys[id];
+}
+
+int push_desc(gdpd_ctx *ctx, int id)
+{
+    if (id >= ctx->d_reps_cap) {
+        int cap = std::max(16, ctx->d_reps_cap * 2);
+        while (cap <= id) cap *= 2;
+        RepDev *nd = nullptr;
+        HIPCHK(hipMalloc((void **)&nd, sizeof(RepDev) * cap));
+        if (ctx->d_reps) {
+            HIPCHK(hipMemcpyAsync(nd, ctx->d_reps, sizeof(RepDev) * ctx->d_reps_cap,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            HIPCHK(hipFree(ctx->d_reps));
+        }
+        ctx->d_reps = nd;
+        ctx->d_reps_cap = cap;
+        // every cached graph and table points at the old array
+        for (Sys *s : ctx->sys)
+            if (s) s->graph.release();
+        ctx->batch_graph.release();
+        ctx->batch.release();
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_reps + id, &ctx->sys[id]->h, sizeof(RepDev), hipMemcpyHostToDevice,
+                          ctx->stream));
+    return GDPD_OK;
+}
+
+// ---- pair coefficient tables -------------------------------------------------
+
+int packed_index(int ti, int tj, int nt)
+{
+    int a = std::min(ti, tj), b = std::max(ti, tj);
+    return ((a - 1) * nt - (a - 1) * (a - 2) / 2 + b - a + 1) - 1;
+}
+
+template <typename real> void build_coef(Sys *s, std::vector<PairCoef<real>> &out, double &rc2max,
+                                         double &rd2max)
+{
+    const int nt = s->desc.n_types;
+    out.assign((size_t)nt * nt, PairCoef<real>{});
+    // force/pair_dpd.cu:577-578 and :315, in the working precision
+    const real dtinv_1_2 = (real)1.0 / std::sqrt((real)s->desc.dt);
+    const real dt_1_2 = std::sqrt((real)2.0 * (real)s->temp) * dtinv_1_2;
+    const real pi = (real)3.14159265358979323846;
+    rc2max = 0.0;
+    rd2max = 0.0;
+    for (int ti = 1; ti <= nt; ++ti)
+        for (int tj = 1; tj <= nt; ++tj) {
+            const int idx = packed_index(ti, tj, nt);
+            PairCoef<real> c{};
+            if (s->style == GDPD_STYLE_MDPD) {
+                c.A = (real)s->coeff[idx * 5 + 0];
+                c.B = (real)s->coeff[idx * 5 + 1];
+                c.gamma = (real)s->coeff[idx * 5 + 2];
+                c.rc = (real)s->coeff[idx * 5 + 3];
+                c.rd = (real)s->coeff[idx * 5 + 4];
+            } else {
+                c.A = (real)s->coeff[idx * 3 + 0];
+                c.B = (real)0;
+                c.gamma = (real)s->coeff[idx * 3 + 1];
+                c.rc = (real)s->coeff[idx * 3 + 2];
+                c.rd = c.rc;
+            }
+            c.rc2 = c.rc * c.rc;
+            c.rcinv = (real)1 / c.rc;
+            c.rdinv = (real)1 / c.rd;
+            c.sig = std::sqrt(c.gamma) * dt_1_2;
+            // force/pair_mdpd.cu:275-276
+            c.rhofac = (real)15.0 / (real)2.0 / pi / (c.rd * c.rd * c.rd);
+            out[(size_t)(ti - 1) * nt + (tj - 1)] = c;
+            rc2max = std::max(rc2max, (double)c.rc2);
+            rd2max = std::max(rd2max, (double)(c.rd * c.rd));
+        }
+}
+
+double max_rc(Sys *s)
+{
+    const int nt = s->desc.n_types, np = nt * (nt + 1) / 2;
+    double m = 0.0;
+    for (int k = 0; k < np; ++k)
+        m = std::max(m, s->style == GDPD_STYLE_MDPD ? s->coeff[k * 5 + 3] : s->coeff[k * 3 + 2]);
+    return m;
+}
+
+// ---- work tables -------------------------------------------------------------
+
+int upload_items(gdpd_ctx *ctx, const std::vector<WorkItem> &v, WorkItem **d)
+{
+    if (v.empty()) return GDPD_OK;
+    HIPCHK(hipMalloc((void **)d, sizeof(WorkItem) * v.size()));
+    HIPCHK(hipMemcpyAsync(*d, v.data(), sizeof(WorkItem) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+    return GDPD_OK;
+}
+
+BrickLaunch brick_params_of(const Sys *s)
+{
+    BrickLaunch b{};
+    b.rmax = (s->h.bdim[0] + 2) * (s->h.bdim[1] + 2) * (s->h.bdim[2] + 2);
+    b.smax = s->h.smax;
+    b.kmax = s->h.kmax;
+    b.ncol = s->h.bdim[0] * s->h.bdim[1];
+    b.ntypes = s->h.ntypes;
+    return b;
+}
+
+void bl_max(BrickLaunch &a, const BrickLaunch &b)
+{
+    a.rmax = std::max(a.rmax, b.rmax);
+    a.smax = std::max(a.smax, b.smax);
+    a.kmax = std::max(a.kmax, b.kmax);
+    a.ncol = std::max(a.ncol, b.ncol);
+    a.ntypes = std::max(a.ntypes, b.ntypes);
+}
+
+int build_tables(gdpd_ctx *ctx, const std::vector<int> &ids, Tables &T)
+{
+    T.release();
+    std::vector<WorkItem> beads, cells, scan, bd, bm;
+    std::vector<int> off(1, 0), reps;
+    for (int id : ids) {
+        Sys *s = ctx->sys[id];
+        const int N = s->h.N, nc = s->h.ncell;
+        for (int k = 0; k < (N + kBlock - 1) / kBlock; ++k) beads.push_back({id, k});
+        for (int k = 0; k < (nc + kBlock - 1) / kBlock; ++k) cells.push_back({id, k});
+        const int nch = (nc + 4 * kBlock - 1) / (4 * kBlock);
+        for (int k = 0; k < nch; ++k) scan.push_back({id, k});
+        off.push_back(off.back() + nch);
+        reps.push_back(id);
+        auto &dst = (s->style == GDPD_STYLE_MDPD) ? bm : bd;
+        for (int k = 0; k < s->h.nbrick; ++k) dst.push_back({id, k});
+        BrickLaunch b = brick_params_of(s);
+        bl_max(s->style == GDPD_STYLE_MDPD ? T.bl_mdpd : T.bl_dpd, b);
+        T.precision = s->h.precision;
+    }
+    int rc;
+    if ((rc = upload_items(ctx, beads, &T.beads))) return rc;
+    if ((rc = upload_items(ctx, cells, &T.cells))) return rc;
+    if ((rc = upload_items(ctx, scan, &T.scan))) return rc;
+    if ((rc = upload_items(ctx, bd, &T.bricks_dpd))) return rc;
+    if ((rc = upload_items(ctx, bm, &T.bricks_mdpd))) return rc;
+    HIPCHK(hipMalloc((void **)&T.rep_chunk_off, sizeof(int) * off.size()));
+    HIPCHK(hipMemcpyAsync(T.rep_chunk_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMalloc((void **)&T.rep_list, sizeof(int) * reps.size()));
+    HIPCHK(hipMemcpyAsync(T.rep_list, reps.data(), sizeof(int) * reps.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMalloc((void **)&T.chunk_sums, sizeof(int) * std::max<size_t>(1, scan.size())));
+    T.n_beads = (int)beads.size();
+    T.n_cells = (int)cells.size();
+    T.n_scan = (int)scan.size();
+    T.n_dpd = (int)bd.size();
+    T.n_mdpd = (int)bm.size();
+    T.n_reps = (int)reps.size();
+    T.key = ids;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+// ---- kernel timing -------------------------------------------------------------
+
+hipEvent_t next_event(gdpd_ctx *ctx)
+{
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+// ---- step sequences ------------------------------------------------------------
+
+int check_launch(gdpd_ctx *ctx)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(ctx, GDPD_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return GDPD_OK;
+}
+
+// rebin: keys for the current positions, then scan/scatter/in-cell sort
+int rebin(gdpd_ctx *ctx, Tables &T)
+{
+    launch_bin_keys(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+    launch_scan(ctx->d_reps, T.scan, T.n_scan, T.rep_chunk_off, T.rep_list, T.n_reps, T.chunk_sums, ctx->stream);
+    launch_scatter(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+    launch_cellsort(ctx->d_reps, T.cells, T.n_cells, ctx->stream);
+    return check_launch(ctx);
+}
+
+int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
+{
+    const bool timed = ctx->timing;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (T.n_mdpd) launch_density(ctx->d_reps, T.bricks_mdpd, T.n_mdpd, T.bl_mdpd, T.precision, ctx->stream);
+    if (timed) {
+        e0 = next_event(ctx);
+        e1 = next_event(ctx);
+        if (!e0 || !e1) return fail(ctx, GDPD_ERR_HIP, "hipEventCreate failed");
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+    }
+    if (T.n_dpd) launch_brick(ctx->d_reps, T.bricks_dpd, T.n_dpd, T.bl_dpd, T.precision, GDPD_STYLE_DPD, mode, ctx->stream);
+    if (T.n_mdpd) launch_brick(ctx->d_reps, T.bricks_mdpd, T.n_mdpd, T.bl_mdpd, T.precision, GDPD_STYLE_MDPD, mode, ctx->stream);
+    if (timed) {
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        ctx->force_launches++;
+    }
+    return check_launch(ctx);
+}
+
+// one fused step for every system in T
+int step_once(gdpd_ctx *ctx, Tables &T, int mode)
+{
+    int rc;
+    if ((rc = force_launch(ctx, T, mode | kModeLogAuto))) return rc;
+    launch_step_end(ctx->d_reps, T.rep_list, T.n_reps, 1, 1, 1, ctx->stream);
+    if (mode & kModeDrift) {
+        launch_scan(ctx->d_reps, T.scan, T.n_scan, T.rep_chunk_off, T.rep_list, T.n_reps, T.chunk_sums, ctx->stream);
+        launch_scatter(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+        launch_cellsort(ctx->d_reps, T.cells, T.n_cells, ctx->stream);
+    }
+    return check_launch(ctx);
+}
+
+int read_state(gdpd_ctx *ctx, Sys *s, RepState *st)
+{
+    HIPCHK(hipMemcpyAsync(st, s->h.state, sizeof(RepState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+int check_flags(gdpd_ctx *ctx, Sys *s)
+{
+    RepState st;
+    int rc = read_state(ctx, s, &st);
+    if (rc) return rc;
+    if (st.flags & kFlagStageOverflow)
+        return fail(ctx, GDPD_ERR_OVERFLOW, "a brick held more beads than its LDS capacity (smax=%d)", s->h.smax);
+    if (st.flags & kFlagListOverflow)
+        return fail(ctx, GDPD_ERR_OVERFLOW, "a bead had more than kmax=%d neighbours in range", s->h.kmax);
+    if (st.flags & kFlagNonFinite)
+        return fail(ctx, GDPD_ERR_NAN, "non-finite or runaway bead coordinates");
+    return GDPD_OK;
+}
+
+// ---- preparation: cell grid, bricks, capacities --------------------------------
+
+int choose_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cell_start)
+{
+    RepDev &h = s->h;
+    // exact region counts of every brick at this state, then a margin for drift
+    int maxreg = 0;
+    double maxdens = 0.0;
+    const double cellvol = h.cs[0] * h.cs[1] * h.cs[2];
+    for (int b = 0; b < h.nbrick; ++b) {
+        const int bz = b % h.nbk[2], by = (b / h.nbk[2]) % h.nbk[1], bx = b / (h.nbk[2] * h.nbk[1]);
+        const int c0[3] = {bx * h.bdim[0], by * h.bdim[1], bz * h.bdim[2]};
+        int ext[3];
+        for (int d = 0; d < 3; ++d) ext[d] = std::min(h.bdim[d], h.nc[d] - c0[d]);
+        int cnt = 0, ncel = 0;
+        for (int rx = -1; rx <= ext[0]; ++rx)
+            for (int ry = -1; ry <= ext[1]; ++ry)
+                for (int rz = -1; rz <= ext[2]; ++rz) {
+                    const int gx = (c0[0] + rx + h.nc[0]) % h.nc[0];
+                    const int gy = (c0[1] + ry + h.nc[1]) % h.nc[1];
+                    const int gz = (c0[2] + rz + h.nc[2]) % h.nc[2];
+                    const int c = (gx * h.nc[1] + gy) * h.nc[2] + gz;
+                    cnt += cell_start[c + 1] - cell_start[c];
+                    ++ncel;
+                }
+        maxreg = std::max(maxreg, cnt);
+        maxdens = std::max(maxdens, cnt / (ncel * cellvol));
+    }
+    h.smax = std::max(64, (int)std::ceil(maxreg * 1.35) + 64);
+    const double rcm = std::sqrt(h.rcmax2);
+    h.kmax = std::max(16, (int)std::ceil(maxdens * 4.0 / 3.0 * M_PI * rcm * rcm * rcm * 1.8) + 16);
+    (void)ctx;
+    return GDPD_OK;
+}
+
+int prepare(gdpd_ctx *ctx, Sys *s)
+{
+    if (!s->uploaded) return fail(ctx, GDPD_ERR_STATE, "upload the system before computing");
+    if (!s->have_pair) return fail(ctx, GDPD_ERR_STATE, "no pair_style set");
+    if (s->prepared) return GDPD_OK;
+    RepDev &h = s->h;
+    const double rcm = max_rc(s);
+    if (!(rcm > 0.0)) return fail(ctx, GDPD_ERR_ARG, "pair cutoff must be positive");
+    for (int d = 0; d < 3; ++d) {
+        h.nc[d] = (int)std::floor(h.L[d] / rcm);
+        if (h.nc[d] < 3)
+            return fail(ctx, GDPD_ERR_ARG,
+                        "box edge %g holds fewer than 3 cells of the cutoff %g (the 27-cell stencil would repeat cells)",
+                        h.L[d], rcm);
+        h.cs[d] = h.L[d] / h.nc[d];
+        h.inv_cs[d] = (double)h.nc[d] / h.L[d];
+    }
+    h.ncell = h.nc[0] * h.nc[1] * h.nc[2];
+    // brick edge: aim at ~200 owned beads per 256-thread workgroup
+    const double rho = h.N / (h.L[0] * h.L[1] * h.L[2]);
+    const double cellvol = h.cs[0] * h.cs[1] * h.cs[2];
+    for (int d = 0; d < 3; ++d) h.bdim[d] = s->tune_b[d];
+    if (s->tune_b[0] <= 0 || s->tune_b[1] <= 0 || s->tune_b[2] <= 0) {
+        const double target = 200.0 / std::max(rho * cellvol, 1e-9);
+        int e = std::max(1, (int)std::floor(std::cbrt(target)));
+        h.bdim[0] = h.bdim[1] = e;
+        h.bdim[2] = std::max(1, (int)std::floor(target / (e * e)));
+        h.bdim[2] = std::min(h.bdim[2], 3 * e);
+    }
+    for (int d = 0; d < 3; ++d) {
+        h.bdim[d] = std::max(1, std::min(h.bdim[d], h.nc[d]));
+        h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
+    }
+    h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
+    if ((h.bdim[0] + 2) * (h.bdim[1] + 2) * (h.bdim[2] + 2) > 4 * kBlock)
+        return fail(ctx, GDPD_ERR_ARG, "brick too large: at most %d region cells", 4 * kBlock);
+
+    int rc;
+    if ((rc = dalloc(ctx, s, &h.cell_start, (size_t)h.ncell + 1))) return rc;
+    if ((rc = dalloc(ctx, s, &h.cell_count, (size_t)h.ncell))) return rc;
+    if ((rc = dalloc(ctx, s, &h.thermo_part, (size_t)h.nbrick * kThermoParts))) return rc;
+    HIPCHK(hipMemsetAsync(h.cell_count, 0, sizeof(int) * h.ncell, ctx->stream));
+
+    // coefficient table in the working precision
+    double rc2max = 0, rd2max = 0;
+    if (h.precision == GDPD_PREC_F64) {
+        std::vector<PairCoef<double>> tab;
+        build_coef<double>(s, tab, rc2max, rd2max);
+        PairCoef<double> *d;
+        if ((rc = dalloc(ctx, s, &d, tab.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(d, tab.data(), sizeof(tab[0]) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+        h.coef = d;
+    } else {
+        std::vector<PairCoef<float>> tab;
+        build_coef<float>(s, tab, rc2max, rd2max);
+        PairCoef<float> *d;
+        if ((rc = dalloc(ctx, s, &d, tab.size()))) return rc;
+        HIPCHK(hipMemcpyAsync(d, tab.data(), sizeof(tab[0]) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+        h.coef = d;
+    }
+    h.rcmax2 = rc2max;
+    h.rdmax2 = rd2max;
+    h.smax = 64;
+    h.kmax = 16;
+    s->prepared = true;
+    const int id = (int)(std::find(ctx->sys.begin(), ctx->sys.end(), s) - ctx->sys.begin());
+    if ((rc = push_desc(ctx, id))) return rc;
+    if ((rc = build_tables(ctx, {id}, s->tab))) return rc;
+
+    // initial binning and physical sort into cell order
+    if ((rc = rebin(ctx, s->tab))) return rc;
+    launch_resort(ctx->d_reps, s->tab.beads, s->tab.n_beads, ctx->stream);
+    if ((rc = check_launch(ctx))) return rc;
+    std::vector<int> cs_host((size_t)h.ncell + 1);
+    HIPCHK(hipMemcpyAsync(cs_host.data(), h.cell_start, sizeof(int) * cs_host.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = choose_capacities(ctx, s, cs_host))) return rc;
+    // shrink the brick until the LDS layout fits
+    for (int guard = 0; guard < 16; ++guard) {
+        BrickLaunch b = brick_params_of(s);
+        if (b.lds_bytes(h.precision, s->style) <= (size_t)ctx->max_lds) break;
+        int dmax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (h.bdim[d] > h.bdim[dmax]) dmax = d;
+        if (h.bdim[dmax] <= 1) return fail(ctx, GDPD_ERR_OVERFLOW, "bead density too high for LDS staging");
+        h.bdim[dmax]--;
+        for (int d = 0; d < 3; ++d) h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
+        h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
+        (void)hipFree(h.thermo_part);
+        s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), (void *)h.thermo_part));
+        if ((rc = dalloc(ctx, s, &h.thermo_part, (size_t)h.nbrick * kThermoParts))) return rc;
+        if ((rc = choose_capacities(ctx, s, cs_host))) return rc;
+    }
+    if ((rc = push_desc(ctx, id))) return rc;
+    if ((rc = build_tables(ctx, {id}, s->tab))) return rc;
+    s->cells_valid = true;
+    s->steps_since_resort = 0;
+    return GDPD_OK;
+}
+
+int ensure_cells(gdpd_ctx *ctx, Sys *s)
+{
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    if (!s->cells_valid) {
+        if ((rc = rebin(ctx, s->tab))) return rc;
+        s->cells_valid = true;
+    }
+    return GDPD_OK;
+}
+
+int set_state_step(gdpd_ctx *ctx, Sys *s, long long step, bool reset_rows)
+{
+    RepState st;
+    int rc = read_state(ctx, s, &st);
+    if (rc) return rc;
+    st.step = step;
+    if (reset_rows) st.log_rows = 0;
+    HIPCHK(hipMemcpyAsync(s->h.state, &st, sizeof(RepState), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+int maybe_resort(gdpd_ctx *ctx, Sys *s, Tables &T, int steps)
+{
+    s->steps_since_resort += steps;
+    const int every = s->resort_every > 0 ? s->resort_every : 16;
+    if (s->steps_since_resort >= every) {
+        launch_resort(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+        s->steps_since_resort = 0;
+        return check_launch(ctx);
+    }
+    return GDPD_OK;
+}
+
+int graph_block(gdpd_ctx *ctx, Graph &G, Tables &T, int steps)
+{
+    if (G.exec && G.steps == steps) {
+        HIPCHK(hipGraphLaunch(G.exec, ctx->stream));
+        return GDPD_OK;
+    }
+    G.release();
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = GDPD_OK;
+    for (int k = 0; k < steps && !rc; ++k) rc = step_once(ctx, T, kModeFinalKick | kModeDrift);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, GDPD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    G.graph = g;
+    HIPCHK(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
+    G.steps = steps;
+    HIPCHK(hipGraphLaunch(G.exec, ctx->stream));
+    return GDPD_OK;
+}
+
+int advance_tables(gdpd_ctx *ctx, Tables &T, Graph &G, int nsteps, int graph_steps,
+                   const std::vector<Sys *> &members)
+{
+    int rc;
+    int done = 0;
+    while (done < nsteps) {
+        // steps until the next re-sort of any member
+        int chunk = nsteps - done;
+        for (Sys *s : members) {
+            const int every = s->resort_every > 0 ? s->resort_every : 16;
+            chunk = std::min(chunk, std::max(1, every - s->steps_since_resort));
+        }
+        const bool use_graph = !ctx->timing && graph_steps > 1;
+        int k = 0;
+        while (k < chunk) {
+            if (use_graph && chunk - k >= graph_steps) {
+                if ((rc = graph_block(ctx, G, T, graph_steps))) return rc;
+                k += graph_steps;
+            } else {
+                if ((rc = step_once(ctx, T, kModeFinalKick | kModeDrift))) return rc;
+                ++k;
+            }
+        }
+        // one physical re-sort covers every member of the launch
+        bool due = false;
+        for (Sys *s : members) {
+            s->steps_since_resort += chunk;
+            const int every = s->resort_every > 0 ? s->resort_every : 16;
+            due = due || s->steps_since_resort >= every;
+        }
+        if (due) {
+            launch_resort(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+            if ((rc = check_launch(ctx))) return rc;
+            for (Sys *s : members) s->steps_since_resort = 0;
+        }
+        done += chunk;
+    }
+    return GDPD_OK;
+}
+
+} // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+
+extern "C" {
+
+int gdpd_create(gdpd_ctx **out, int device)
+{
+    if (!out) return GDPD_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GDPD_ERR_HIP;
+    if (device < 0 || device >= n) return GDPD_ERR_ARG;
+    gdpd_ctx *ctx = new gdpd_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return GDPD_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
+        ctx->max_lds = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
+    *out = ctx;
+    return GDPD_OK;
+}
+
+int gdpd_destroy(gdpd_ctx *ctx)
+{
+    if (!ctx) return GDPD_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (size_t i = 0; i < ctx->sys.size(); ++i)
+        if (ctx->sys[i]) gdpd_sys_destroy(ctx, (int)i);
+    ctx->batch.release();
+    ctx->batch_graph.release();
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->d_reps) (void)hipFree(ctx->d_reps);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GDPD_OK;
+}
+
+const char *gdpd_last_error(const gdpd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t gdpd_system_bytes(const gdpd_system_desc *d, int n_bonds)
+{
+    if (!d || d->n_beads <= 0) return 0;
+    const int64_t N = d->n_beads;
+    double vol = 1.0;
+    for (int k = 0; k < 3; ++k) vol *= (d->hi[k] - d->lo[k]);
+    const int64_t ncell = (int64_t)std::max(1.0, vol / std::max(d->global_cut * d->global_cut * d->global_cut, 1e-3));
+    // state x2 (96) + force/pe/bpe/vir/rho (96) + ints (36) + type
+    return N * 233 + ncell * 8 + (int64_t)n_bonds * 16 + (1 << 20);
+}
+
+int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
+{
+    if (!ctx || !d || !sys_out) return fail(ctx, GDPD_ERR_ARG, "null argument");
+    if (d->n_beads <= 0) return fail(ctx, GDPD_ERR_ARG, "n_beads must be positive");
+    if (d->n_types < 1 || d->n_types > kMaxTypes)
+        return fail(ctx, GDPD_ERR_ARG, "n_types must be in 1..%d", kMaxTypes);
+    for (int k = 0; k < 3; ++k)
+        if (!(d->hi[k] > d->lo[k])) return fail(ctx, GDPD_ERR_ARG, "box hi must exceed lo");
+    if (!(d->dt > 0.0)) return fail(ctx, GDPD_ERR_ARG, "timestep must be positive");
+    if (d->precision != GDPD_PREC_F32 && d->precision != GDPD_PREC_F64)
+        return fail(ctx, GDPD_ERR_ARG, "unknown precision %d", d->precision);
+    HIPCHK(hipSetDevice(ctx->device));
+    Sys *s = new Sys();
+    s->desc = *d;
+    RepDev &h = s->h;
+    const int N = d->n_beads;
+    h.N = N;
+    h.ntypes = d->n_types;
+    h.precision = d->precision;
+    h.dt = d->dt;
+    h.log_every = d->log_every > 0 ? d->log_every : 0;
+    for (int k = 0; k < 3; ++k) {
+        h.lo[k] = d->lo[k];
+        h.L[k] = d->hi[k] - d->lo[k];
+        h.hL[k] = 0.5 * h.L[k];
+    }
+    int id = -1;
+    for (size_t i = 0; i < ctx->sys.size(); ++i)
+        if (!ctx->sys[i]) {
+            id = (int)i;
+            break;
+        }
+    if (id < 0) {
+        id = (int)ctx->sys.size();
+        ctx->sys.push_back(nullptr);
+    }
+    ctx->sys[id] = s;
+    int rc = GDPD_OK;
+#define ALLOC(p, n)                                    \
+    if ((rc = dalloc(ctx, s, &(p), (size_t)(n)))) {    \
+        gdpd_sys_destroy(ctx, id);                     \
+        return rc;                                     \
+    }
+    ALLOC(h.x[0], 3 * (size_t)N);
+    ALLOC(h.x[1], 3 * (size_t)N);
+    ALLOC(h.v[0], 3 * (size_t)N);
+    ALLOC(h.v[1], 3 * (size_t)N);
+    ALLOC(h.id, N);
+    ALLOC(h.type, N);
+    ALLOC(h.img, N);
+    ALLOC(h.perm, N);
+    ALLOC(h.key, N);
+    ALLOC(h.rank, N);
+    ALLOC(h.slot_of_id, N);
+    ALLOC(h.force, 3 * (size_t)N);
+    ALLOC(h.pe, N);
+    ALLOC(h.bpe, N);
+    ALLOC(h.vir, 6 * (size_t)N);
+    ALLOC(h.rho, N);
+    ALLOC(h.iscratch, 3 * (size_t)N);
+    ALLOC(h.state, 1);
+    h.thermo_cap = 1024;
+    ALLOC(h.thermo_rows, (size_t)h.thermo_cap * 12);
+    // empty bond CSR (bond_start[id] == 0 for all ids)
+    ALLOC(h.bond_start, (size_t)N + 1);
+#undef ALLOC
+    h.scratch = nullptr;
+    HIPCHK(hipMemsetAsync(h.bond_start, 0, sizeof(int) * ((size_t)N + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(h.force, 0, sizeof(double) * 3 * (size_t)N, ctx->stream));
+    HIPCHK(hipMemsetAsync(h.pe, 0, sizeof(double) * (size_t)N, ctx->stream));
+    HIPCHK(hipMemsetAsync(h.bpe, 0, sizeof(double) * (size_t)N, ctx->stream));
+    HIPCHK(hipMemsetAsync(h.vir, 0, sizeof(double) * 6 * (size_t)N, ctx->stream));
+    HIPCHK(hipMemsetAsync(h.rho, 0, sizeof(double) * (size_t)N, ctx->stream));
+    RepState st{};
+    HIPCHK(hipMemcpyAsync(h.state, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = push_desc(ctx, id))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *sys_out = id;
+    return GDPD_OK;
+}
+
+int gdpd_sys_destroy(gdpd_ctx *ctx, int id)
+{
+    Sys *s = get_sys(ctx, id);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", id);
+    (void)hipStreamSynchronize(ctx->stream);
+    s->tab.release();
+    s->graph.release();
+    for (void *p : s->allocs) (void)hipFree(p);
+    void *nl[] = {s->nl_binof, s->nl_count, s->nl_start, s->nl_fill, s->nl_content, s->nl_n,
+                  s->nl_neigh, s->nl_over, s->nl_flag, s->nl_last};
+    for (void *p : nl)
+        if (p) (void)hipFree(p);
+    delete s;
+    ctx->sys[id] = nullptr;
+    ctx->batch.release();
+    ctx->batch_graph.release();
+    return GDPD_OK;
+}
+
+int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const double *pos,
+                const double *vel, const double *masses)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !id || !type || !pos || !masses) return fail(ctx, GDPD_ERR_ARG, "bad upload arguments");
+    RepDev &h = s->h;
+    const int N = h.N;
+    std::vector<double> X(3 * (size_t)N), V(3 * (size_t)N, 0.0);
+    std::vector<int> ids(N), img(N, pack_img(0, 0, 0)), soi(N, -1), perm(N);
+    std::vector<unsigned char> ty(N);
+    for (int k = 0; k < N; ++k) {
+        if (id[k] < 1 || id[k] > N) return fail(ctx, GDPD_ERR_ARG, "atom id %d outside 1..%d", id[k], N);
+        if (soi[id[k] - 1] >= 0) return fail(ctx, GDPD_ERR_ARG, "duplicate atom id %d", id[k]);
+        if (type[k] < 1 || type[k] > h.ntypes)
+            return fail(ctx, GDPD_ERR_ARG, "atom type %d outside 1..%d", type[k], h.ntypes);
+        soi[id[k] - 1] = k;
+        ids[k] = id[k];
+        ty[k] = (unsigned char)type[k];
+        perm[k] = k;
+        for (int d = 0; d < 3; ++d) {
+            double x = pos[3 * (size_t)k + d];
+            if (!std::isfinite(x)) return fail(ctx, GDPD_ERR_NAN, "non-finite position of atom %d", id[k]);
+            // system/read_data.cu:191-197
+            const double lo = h.lo[d], hi = h.lo[d] + h.L[d];
+            while (x >= hi || x < lo) x -= (hi - lo) * (double)((x >= hi) - (x < lo));
+            X[(size_t)d * N + k] = x;
+            if (vel) V[(size_t)d * N + k] = vel[3 * (size_t)k + d];
+        }
+    }
+    s->masses.assign(masses, masses + h.ntypes);
+    for (int t = 0; t < h.ntypes; ++t) {
+        if (!(masses[t] > 0.0)) return fail(ctx, GDPD_ERR_ARG, "mass of type %d must be positive", t + 1);
+        h.mass[t] = masses[t];
+        h.minv[t] = 1.0 / masses[t];
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    RepState st{};
+    HIPCHK(hipMemcpyAsync(h.state, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.x[0], X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.x[1], X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.v[0], V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.v[1], V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.id, ids.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.type, ty.data(), N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.img, img.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.perm, perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.slot_of_id, soi.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    s->uploaded = true;
+    s->cells_valid = false;
+    s->midstep = false;
+    s->nl_built = false;
+    int rc = push_desc(ctx, sid);
+    if (rc) return rc;
+    if (s->prepared) {
+        // re-sort the fresh state into cell order
+        if ((rc = rebin(ctx, s->tab))) return rc;
+        launch_resort(ctx->d_reps, s->tab.beads, s->tab.n_beads, ctx->stream);
+        if ((rc = check_launch(ctx))) return rc;
+        s->cells_valid = true;
+        s->steps_since_resort = 0;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+int gdpd_download(gdpd_ctx *ctx, int sid, double *pos, double *vel, int *image, double *force)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !s->uploaded) return fail(ctx, GDPD_ERR_STATE, "system %d has no state", sid);
+    RepDev &h = s->h;
+    const int N = h.N;
+    RepState st;
+    int rc = read_state(ctx, s, &st);
+    if (rc) return rc;
+    std::vector<int> ids(N);
+    HIPCHK(hipMemcpy(ids.data(), h.id, sizeof(int) * N, hipMemcpyDeviceToHost));
+    auto fetch3 = [&](const double *src, double *dst) -> int {
+        std::vector<double> tmp(3 * (size_t)N);
+        HIPCHK(hipMemcpy(tmp.data(), src, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k)
+            for (int d = 0; d < 3; ++d) dst[3 * (size_t)(ids[k] - 1) + d] = tmp[(size_t)d * N + k];
+        return GDPD_OK;
+    };
+    if (pos && (rc = fetch3(h.x[st.cur], pos))) return rc;
+    if (vel && (rc = fetch3(h.v[st.cur], vel))) return rc;
+    if (force && (rc = fetch3(h.force, force))) return rc;
+    if (image) {
+        std::vector<int> im(N);
+        HIPCHK(hipMemcpy(im.data(), h.img, sizeof(int) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) {
+            int a, b, c;
+            unpack_img(im[k], a, b, c);
+            image[3 * (size_t)(ids[k] - 1) + 0] = a;
+            image[3 * (size_t)(ids[k] - 1) + 1] = b;
+            image[3 * (size_t)(ids[k] - 1) + 2] = c;
+        }
+    }
+    return GDPD_OK;
+}
+
+int gdpd_download_pe_virial(gdpd_ctx *ctx, int sid, double *pe, double *virial, double *rho)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !s->uploaded) return fail(ctx, GDPD_ERR_STATE, "system %d has no state", sid);
+    RepDev &h = s->h;
+    const int N = h.N;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> ids(N);
+    HIPCHK(hipMemcpy(ids.data(), h.id, sizeof(int) * N, hipMemcpyDeviceToHost));
+    std::vector<double> tmp(6 * (size_t)N);
+    if (pe) {
+        HIPCHK(hipMemcpy(tmp.data(), h.pe, sizeof(double) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) pe[ids[k] - 1] = tmp[k];
+    }
+    if (rho) {
+        HIPCHK(hipMemcpy(tmp.data(), h.rho, sizeof(double) * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k) rho[ids[k] - 1] = tmp[k];
+    }
+    if (virial) {
+        HIPCHK(hipMemcpy(tmp.data(), h.vir, sizeof(double) * 6 * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k)
+            for (int c = 0; c < 6; ++c) virial[6 * (size_t)(ids[k] - 1) + c] = tmp[(size_t)c * N + k];
+    }
+    return GDPD_OK;
+}
+
+static int set_pair(gdpd_ctx *ctx, int sid, int style, double temp, double rc, double rd,
+                    uint32_t seed, const double *coeff)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !coeff) return fail(ctx, GDPD_ERR_ARG, "bad pair arguments");
+    if (s->prepared) return fail(ctx, GDPD_ERR_STATE, "pair style is fixed once the system has run");
+    const int nt = s->desc.n_types, np = nt * (nt + 1) / 2, w = (style == GDPD_STYLE_MDPD) ? 5 : 3;
+    // force/pair_dpd.cu:78-81,140-143; pair_mdpd.cu:81-89,154-162
+    if (rc > s->desc.global_cut)
+        return fail(ctx, GDPD_ERR_ARG, "global rc should not be bigger than the global_cut (neighbor)");
+    if (style == GDPD_STYLE_MDPD && rd > rc) return fail(ctx, GDPD_ERR_ARG, "rd should not be bigger than rc");
+    for (int k = 0; k < np; ++k) {
+        const double rck = (style == GDPD_STYLE_MDPD) ? coeff[k * 5 + 3] : coeff[k * 3 + 2];
+        if (!(rck > 0.0)) return fail(ctx, GDPD_ERR_ARG, "pair cutoff of pair %d must be positive", k);
+        if (rck > s->desc.global_cut)
+            return fail(ctx, GDPD_ERR_ARG, "global rc should not be bigger than the global_cut (neighbor)");
+        if (style == GDPD_STYLE_MDPD && coeff[k * 5 + 4] > rck)
+            return fail(ctx, GDPD_ERR_ARG, "rd should not be bigger than rc");
+        if (style == GDPD_STYLE_MDPD && !(coeff[k * 5 + 4] > 0.0))
+            return fail(ctx, GDPD_ERR_ARG, "rd must be positive");
+        const double g = (style == GDPD_STYLE_MDPD) ? coeff[k * 5 + 2] : coeff[k * 3 + 1];
+        if (g < 0.0) return fail(ctx, GDPD_ERR_ARG, "gamma must be non-negative");
+    }
+    if (temp < 0.0) return fail(ctx, GDPD_ERR_ARG, "temperature must be non-negative");
+    s->style = style;
+    s->temp = temp;
+    s->rc = rc;
+    s->rd = rd;
+    s->seed = seed;
+    s->coeff.assign(coeff, coeff + (size_t)np * w);
+    s->have_pair = true;
+    s->h.style = style;
+    s->h.seed = seed;
+    return push_desc(ctx, sid);
+}
+
+int gdpd_set_pair_dpd(gdpd_ctx *ctx, int sid, double temp, double rc, uint32_t seed, const double *coeff)
+{
+    return set_pair(ctx, sid, GDPD_STYLE_DPD, temp, rc, rc, seed, coeff);
+}
+
+int gdpd_set_pair_mdpd(gdpd_ctx *ctx, int sid, double temp, double rc, double rd, uint32_t seed,
+                       const double *coeff)
+{
+    return set_pair(ctx, sid, GDPD_STYLE_MDPD, temp, rc, rd, seed, coeff);
+}
+
+int gdpd_set_bonds(gdpd_ctx *ctx, int sid, int n_bonds, int n_bond_types, const int *bondlist,
+                   const double *bond_coeff)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || n_bonds < 0 || (n_bonds > 0 && (!bondlist || !bond_coeff)))
+        return fail(ctx, GDPD_ERR_ARG, "bad bond arguments");
+    if (n_bond_types < 0 || n_bond_types > kMaxBondTypes)
+        return fail(ctx, GDPD_ERR_ARG, "n_bond_types must be in 0..%d", kMaxBondTypes);
+    RepDev &h = s->h;
+    const int N = h.N;
+    std::vector<int> deg(N + 1, 0);
+    for (int b = 0; b < n_bonds; ++b) {
+        const int t = bondlist[3 * b], i = bondlist[3 * b + 1], j = bondlist[3 * b + 2];
+        if (t < 1 || t > n_bond_types) return fail(ctx, GDPD_ERR_ARG, "bond %d: type exceeds n_bond_types", b);
+        if (i < 1 || i > N || j < 1 || j > N) return fail(ctx, GDPD_ERR_ARG, "bond %d: atom id out of range", b);
+        deg[i]++;
+        deg[j]++;
+    }
+    std::vector<int> start(N + 1, 0);
+    for (int k = 0; k < N; ++k) start[k + 1] = start[k] + deg[k + 1];
+    std::vector<int> fillp(start.begin(), start.end() - 1), partner(std::max(1, 2 * n_bonds));
+    std::vector<unsigned char> bt(std::max(1, 2 * n_bonds));
+    for (int b = 0; b < n_bonds; ++b) {
+        const int t = bondlist[3 * b] - 1, i = bondlist[3 * b + 1], j = bondlist[3 * b + 2];
+        partner[fillp[i - 1]] = j;
+        bt[fillp[i - 1]++] = (unsigned char)t;
+        partner[fillp[j - 1]] = i;
+        bt[fillp[j - 1]++] = (unsigned char)t;
+    }
+    for (int t = 0; t < n_bond_types; ++t) {
+        h.bond_k[t] = bond_coeff[2 * t];
+        h.bond_r0[t] = bond_coeff[2 * t + 1];
+    }
+    int rc;
+    if ((rc = dalloc(ctx, s, &h.bond_partner, partner.size()))) return rc;
+    if ((rc = dalloc(ctx, s, &h.bond_type, bt.size()))) return rc;
+    HIPCHK(hipMemcpyAsync(h.bond_start, start.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.bond_partner, partner.data(), sizeof(int) * partner.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.bond_type, bt.data(), bt.size(), hipMemcpyHostToDevice, ctx->stream));
+    h.nbonds = n_bonds;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return push_desc(ctx, sid);
+}
+
+// ---- reference neighbour list ---------------------------------------------------
+
+int gdpd_neigh_build(gdpd_ctx *ctx, int sid)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !s->uploaded) return fail(ctx, GDPD_ERR_STATE, "system %d has no state", sid);
+    RepDev &h = s->h;
+    const int N = h.N;
+    const double bin = s->desc.global_cut + s->desc.skin;
+    if (!(bin > 0.0)) return fail(ctx, GDPD_ERR_ARG, "global_cut + skin must be positive");
+    if (!s->nl_n) {
+        // force/neigh_list.cu:14-17, 64-68
+        const double sphere = 4.0 / 3.0 * M_PI * std::pow(bin, 3.0);
+        int nmax = (int)(s->desc.max_rho * sphere) + 1;
+        s->nl_nmax = ((nmax + 15) / 16) * 16;
+        long long nbins = 1;
+        for (int d = 0; d < 3; ++d) {
+            s->nl_nb[d] = (int)(h.L[d] / bin);
+            if (s->nl_nb[d] < 3) return fail(ctx, GDPD_ERR_ARG, "fewer than 3 neighbour bins along axis %d", d);
+            nbins *= s->nl_nb[d];
+        }
+        HIPCHK(hipMalloc((void **)&s->nl_binof, sizeof(int) * N));
+        HIPCHK(hipMalloc((void **)&s->nl_count, sizeof(int) * nbins));
+        HIPCHK(hipMalloc((void **)&s->nl_start, sizeof(int) * (nbins + 1)));
+        HIPCHK(hipMalloc((void **)&s->nl_fill, sizeof(int) * nbins));
+        HIPCHK(hipMalloc((void **)&s->nl_content, sizeof(int) * N));
+        HIPCHK(hipMalloc((void **)&s->nl_n, sizeof(int) * N));
+        HIPCHK(hipMalloc((void **)&s->nl_neigh, sizeof(int) * (size_t)N * s->nl_nmax));
+        HIPCHK(hipMalloc((void **)&s->nl_over, sizeof(int)));
+        HIPCHK(hipMalloc((void **)&s->nl_flag, sizeof(int)));
+        HIPCHK(hipMalloc((void **)&s->nl_last, sizeof(double) * 3 * N));
+    }
+    const long long nbins = (long long)s->nl_nb[0] * s->nl_nb[1] * s->nl_nb[2];
+    HIPCHK(hipMemsetAsync(s->nl_count, 0, sizeof(int) * nbins, ctx->stream));
+    HIPCHK(hipMemsetAsync(s->nl_fill, 0, sizeof(int) * nbins, ctx->stream));
+    HIPCHK(hipMemsetAsync(s->nl_over, 0, sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(s->nl_neigh, 0xff, sizeof(int) * (size_t)N * s->nl_nmax, ctx->stream));
+    RepDev *dev = ctx->d_reps + sid;
+    launch_nl_bin(dev, N, bin, s->nl_nb, s->nl_binof, s->nl_count, ctx->stream);
+    std::vector<int> cnt(nbins), start(nbins + 1, 0);
+    HIPCHK(hipMemcpyAsync(cnt.data(), s->nl_count, sizeof(int) * nbins, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (long long b = 0; b < nbins; ++b) start[b + 1] = start[b] + cnt[b];
+    HIPCHK(hipMemcpyAsync(s->nl_start, start.data(), sizeof(int) * (nbins + 1), hipMemcpyHostToDevice, ctx->stream));
+    launch_nl_fill(dev, N, s->nl_binof, s->nl_start, s->nl_fill, s->nl_content, ctx->stream);
+    launch_nl_build(dev, N, bin, s->nl_nb, s->nl_binof, s->nl_start, s->nl_content, s->nl_nmax,
+                    s->nl_n, s->nl_neigh, s->nl_over, s->nl_last, ctx->stream);
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    s->nl_built = true;
+    return GDPD_OK;
+}
+
+int gdpd_neigh_check(gdpd_ctx *ctx, int sid, int *flag)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !flag) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (!s->nl_built) return fail(ctx, GDPD_ERR_STATE, "no neighbour list built");
+    HIPCHK(hipMemsetAsync(s->nl_flag, 0, sizeof(int), ctx->stream));
+    launch_nl_check(ctx->d_reps + sid, s->h.N, s->desc.skin, s->nl_last, s->nl_flag, ctx->stream);
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(flag, s->nl_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+int gdpd_neigh_download(gdpd_ctx *ctx, int sid, int *n_neigh, int *neigh, int n_max)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !n_neigh || !neigh || n_max <= 0) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (!s->nl_built) return fail(ctx, GDPD_ERR_STATE, "no neighbour list built");
+    const int N = s->h.N, M = s->nl_nmax;
+    std::vector<int> cnt(N), lst((size_t)N * M);
+    int over = 0;
+    HIPCHK(hipMemcpy(cnt.data(), s->nl_n, sizeof(int) * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lst.data(), s->nl_neigh, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&over, s->nl_over, sizeof(int), hipMemcpyDeviceToHost));
+    bool trunc = false;
+    for (int i = 0; i < N; ++i) {
+        std::vector<int> row(cnt[i]);
+        for (int k = 0; k < cnt[i]; ++k) row[k] = lst[(size_t)k * N + i];
+        std::sort(row.begin(), row.end());
+        const int m = std::min((int)row.size(), n_max);
+        if ((int)row.size() > n_max) trunc = true;
+        n_neigh[i] = m;
+        for (int k = 0; k < m; ++k) neigh[(size_t)i * n_max + k] = row[k];
+    }
+    if (over) return fail(ctx, GDPD_ERR_OVERFLOW, "a bead had more than n_max_neigh=%d list neighbours", M);
+    if (trunc) return fail(ctx, GDPD_ERR_OVERFLOW, "row stride %d too small", n_max);
+    return GDPD_OK;
+}
+
+// ---- plugin-hook path -------------------------------------------------------------
+
+int gdpd_pair_compute(gdpd_ctx *ctx, int sid, uint32_t step, int log)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before using hooks");
+    int rc = ensure_cells(ctx, s);
+    if (rc) return rc;
+    if ((rc = set_state_step(ctx, s, step, false))) return rc;
+    if ((rc = force_launch(ctx, s->tab, kModeForceOnly | (log ? kModeLogForce : 0)))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return check_flags(ctx, s);
+}
+
+int gdpd_bond_compute(gdpd_ctx *ctx, int sid, uint32_t step, int log)
+{
+    (void)step;
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    if (s->h.nbonds == 0) return GDPD_OK;
+    launch_bond_hook(ctx->d_reps, s->tab.beads, s->tab.n_beads, log, ctx->stream);
+    if ((rc = check_launch(ctx))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+static int nve_hook(gdpd_ctx *ctx, int sid, int initial)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before using hooks");
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    launch_nve(ctx->d_reps, s->tab.beads, s->tab.n_beads, initial, ctx->stream);
+    if ((rc = check_launch(ctx))) return rc;
+    if (initial) s->cells_valid = false;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return initial ? check_flags(ctx, s) : GDPD_OK;
+}
+
+int gdpd_nve_initial(gdpd_ctx *ctx, int sid) { return nve_hook(ctx, sid, 1); }
+int gdpd_nve_final(gdpd_ctx *ctx, int sid) { return nve_hook(ctx, sid, 0); }
+
+int gdpd_thermo(gdpd_ctx *ctx, int sid, double *out)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !out) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    const int n = s->tab.n_beads;
+    double *part = nullptr;
+    HIPCHK(hipMalloc((void **)&part, sizeof(double) * kThermoParts * n));
+    launch_thermo_arrays(ctx->d_reps, s->tab.beads, n, part, ctx->stream);
+    std::vector<double> hp((size_t)kThermoParts * n);
+    hipError_t e = hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return fail(ctx, GDPD_ERR_HIP, "thermo reduction failed: %s", hipGetErrorString(e));
+    double tot[kThermoParts] = {0};
+    for (int b = 0; b < n; ++b)
+        for (int k = 0; k < kThermoParts; ++k) tot[k] += hp[(size_t)b * kThermoParts + k];
+    const RepDev &h = s->h;
+    const double vol = h.L[0] * h.L[1] * h.L[2];
+    out[0] = (2.0 * tot[0]) / ((3.0 * h.N - 3.0) * 1.0);
+    out[1] = tot[1];
+    out[2] = tot[2];
+    for (int k = 0; k < 6; ++k) out[4 + k] = tot[3 + k] / vol;
+    out[3] = (out[4] + out[7] + out[9]) / 3.0;
+    out[10] = tot[0];
+    out[11] = -1.0;
+    return GDPD_OK;
+}
+
+// ---- fused path ----------------------------------------------------------------------
+
+int gdpd_zero_step(gdpd_ctx *ctx, int sid)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    int rc = ensure_cells(ctx, s);
+    if (rc) return rc;
+    if ((rc = set_state_step(ctx, s, 0, true))) return rc;
+    if ((rc = step_once(ctx, s->tab, kModeDrift))) return rc;
+    if ((rc = maybe_resort(ctx, s, s->tab, 1))) return rc;
+    s->midstep = true;
+    return GDPD_OK;
+}
+
+int gdpd_advance(gdpd_ctx *ctx, int sid, int nsteps)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "call gdpd_zero_step first");
+    if (nsteps <= 0) return GDPD_OK;
+    const int gs = s->graph_steps > 0 ? s->graph_steps : 8;
+    return advance_tables(ctx, s->tab, s->graph, nsteps, gs, {s});
+}
+
+int gdpd_finish(gdpd_ctx *ctx, int sid)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "call gdpd_zero_step first");
+    int rc = step_once(ctx, s->tab, kModeFinalKick | kModeStoreForce);
+    if (rc) return rc;
+    s->midstep = false;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return check_flags(ctx, s);
+}
+
+int gdpd_run(gdpd_ctx *ctx, int sid, int nsteps)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (nsteps < 0) return fail(ctx, GDPD_ERR_ARG, "nsteps must be >= 0");
+    int rc = ensure_cells(ctx, s);
+    if (rc) return rc;
+    if (nsteps == 0) {
+        if ((rc = set_state_step(ctx, s, 0, true))) return rc;
+        if ((rc = step_once(ctx, s->tab, kModeStoreForce))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return check_flags(ctx, s);
+    }
+    if ((rc = gdpd_zero_step(ctx, sid))) return rc;
+    if ((rc = gdpd_advance(ctx, sid, nsteps - 1))) return rc;
+    return gdpd_finish(ctx, sid);
+}
+
+int gdpd_thermo_rows(gdpd_ctx *ctx, int sid, double *out, int max_rows, int *n_rows)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !n_rows) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    RepState st;
+    int rc = read_state(ctx, s, &st);
+    if (rc) return rc;
+    const int cap = s->h.thermo_cap;
+    const int have = std::min(st.log_rows, cap);
+    const int first = st.log_rows - have;
+    std::vector<double> rows((size_t)cap * 12);
+    HIPCHK(hipMemcpy(rows.data(), s->h.thermo_rows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
+    const int n = std::min(have, max_rows);
+    for (int k = 0; k < n && out; ++k) {
+        const int r = (first + k) % cap;
+        memcpy(out + (size_t)k * GDPD_THERMO_LEN, rows.data() + (size_t)r * 12, sizeof(double) * 12);
+    }
+    *n_rows = n;
+    return GDPD_OK;
+}
+
+int gdpd_get_step(gdpd_ctx *ctx, int sid, int64_t *step)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !step) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    RepState st;
+    int rc = read_state(ctx, s, &st);
+    if (rc) return rc;
+    *step = st.step;
+    return GDPD_OK;
+}
+
+int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n, int nsteps)
+{
+    if (!ctx || !systems || n <= 0) return fail(ctx, GDPD_ERR_ARG, "bad batch arguments");
+    std::vector<int> ids(systems, systems + n);
+    std::vector<Sys *> members;
+    int prec = -1;
+    for (int id : ids) {
+        Sys *s = get_sys(ctx, id);
+        if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", id);
+        if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "system %d: call gdpd_zero_step first", id);
+        if (prec >= 0 && s->h.precision != prec) return fail(ctx, GDPD_ERR_ARG, "a batch must share one precision");
+        prec = s->h.precision;
+        members.push_back(s);
+    }
+    int rc;
+    if (ctx->batch.key != ids) {
+        ctx->batch_graph.release();
+        if ((rc = build_tables(ctx, ids, ctx->batch))) return rc;
+    }
+    int gs = 8;
+    for (Sys *s : members)
+        if (s->graph_steps > 0) gs = s->graph_steps;
+    return advance_tables(ctx, ctx->batch, ctx->batch_graph, nsteps, gs, members);
+}
+
+int gdpd_set_tuning(gdpd_ctx *ctx, int sid, int bx, int by, int bz, int resort_every, int graph_steps)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (s->prepared && (bx || by || bz))
+        return fail(ctx, GDPD_ERR_STATE, "brick shape is fixed once the system has run");
+    s->tune_b[0] = bx;
+    s->tune_b[1] = by;
+    s->tune_b[2] = bz;
+    if (resort_every >= 0) s->resort_every = resort_every;
+    if (graph_steps >= 0) {
+        s->graph_steps = graph_steps;
+        s->graph.release();
+    }
+    return GDPD_OK;
+}
+
+int gdpd_get_tuning(gdpd_ctx *ctx, int sid, int *out8)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !out8) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    out8[0] = s->h.bdim[0];
+    out8[1] = s->h.bdim[1];
+    out8[2] = s->h.bdim[2];
+    out8[3] = s->h.smax;
+    out8[4] = s->h.kmax;
+    out8[5] = s->resort_every > 0 ? s->resort_every : 16;
+    out8[6] = s->graph_steps > 0 ? s->graph_steps : 8;
+    out8[7] = s->h.nbrick;
+    return GDPD_OK;
+}
+
+int gdpd_kernel_timing(gdpd_ctx *ctx, int on)
+{
+    if (!ctx) return GDPD_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->timing = on != 0;
+    ctx->ev_used = 0;
+    ctx->force_launches = 0;
+    return GDPD_OK;
+}
+
+int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launches, double *total_ms)
+{
+    if (!ctx) return GDPD_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double sum = 0.0;
+    for (size_t k = 0; k + 1 < ctx->ev_used; k += 2) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[k], ctx->ev_pool[k + 1]));
+        sum += ms;
+    }
+    double tot = 0.0;
+    if (ctx->ev_used >= 2) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->ev_pool[0], ctx->ev_pool[ctx->ev_used - 1]));
+        tot = ms;
+    }
+    if (force_ms) *force_ms = sum;
+    if (force_launches) *force_launches = ctx->force_launches;
+    if (total_ms) *total_ms = tot;
+    return GDPD_OK;
+}
+
+int gdpd_sync(gdpd_ctx *ctx)
+{
+    if (!ctx) return GDPD_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+} // extern "C"

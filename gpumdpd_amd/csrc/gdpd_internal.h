@@ -1,0 +1,105 @@
+// gdpd_internal.h -- device-side descriptors shared by the kernels and the host
+// runtime of libgdpd.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gdpd {
+
+constexpr int kMaxTypes = 12;      // the reference's 256-entry constant tables cap DPD at 12 types
+constexpr int kMaxBondTypes = 16;
+constexpr int kBlock = 256;        // threads per workgroup for every kernel
+constexpr int kThermoParts = 10;   // ke, pair pe, bond pe, p[6], unused
+constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// device flag bits (RepState::flags)
+constexpr int kFlagStageOverflow = 1;
+constexpr int kFlagListOverflow = 2;
+constexpr int kFlagNonFinite = 4;
+constexpr int kFlagNeighOverflow = 8;
+
+// step-kernel mode bits
+constexpr int kModeFinalKick = 1;  // v += dt/2 F/m before thermo (not at step 0)
+constexpr int kModeDrift = 2;      // v += dt/2 F/m; x += dt v; rebin (not at the last step)
+constexpr int kModeStoreForce = 4; // keep F in the force array
+constexpr int kModeForceOnly = 8;  // hook path: write F (and pe/virial), no integration
+constexpr int kModeLogAuto = 16;   // log step when step % log_every == 0
+constexpr int kModeLogForce = 32;  // log step regardless of the step number
+
+// Coefficients of one (type_i, type_j) pair, precomputed on the host in the
+// working precision.  sig is sqrt(gamma) * sqrt(2T)/sqrt(dt) evaluated in the
+// order of force/pair_dpd.cu:315,577-578.
+template <typename real> struct PairCoef {
+    real A, B, gamma, rc, rc2, rcinv, rd, rdinv, sig, rhofac;
+};
+
+// Mutable per-replica scalars that live on the device so a captured graph can
+// be replayed: the step counter, the state-buffer parity and error flags.
+struct RepState {
+    long long step;   // step number of the next force evaluation
+    int cur;          // which of the two x/v buffers is current
+    int flags;        // kFlag* bits, sticky
+    int log_rows;     // thermo records written in this run
+    int pad;
+};
+
+// One replica as the kernels see it.  All bead arrays are in "slot" order; the
+// cell lists map cell -> sorted positions -> slots through perm.
+struct RepDev {
+    double lo[3], L[3], hL[3];
+    double cs[3], inv_cs[3];   // cell edge per axis (L / nc) and its inverse
+    int nc[3], ncell;
+    int bdim[3], nbk[3], nbrick;
+    int N, ntypes, style, precision;
+    int smax, kmax;            // LDS capacities: staged beads, list entries per bead
+    double rcmax2, rdmax2;     // largest rc^2 and rd^2 over type pairs
+    int log_every;
+    unsigned seed;
+    double dt;
+    double mass[kMaxTypes], minv[kMaxTypes];
+    double bond_k[kMaxBondTypes], bond_r0[kMaxBondTypes];
+
+    double *x[2], *v[2];       // SoA, 3N each: [X..., Y..., Z...]
+    int *id;                   // 1-based ids
+    unsigned char *type;       // 1-based types
+    int *img;                  // packed image counts, 10 bits per axis (offset 512)
+    int *perm;                 // sorted position -> slot
+    int *cell_start;           // ncell + 1
+    int *cell_count;           // ncell, zero between steps
+    int *key, *rank;           // per slot: cell of the new position, rank in that cell
+    int *slot_of_id;           // id-1 -> slot
+    double *force;             // SoA 3N
+    double *pe, *vir;          // N and SoA 6N (log steps)
+    double *bpe;               // N: per-bead half of its bonds' energy (log steps)
+    double *rho;               // N (mDPD)
+    int nbonds;
+    int *bond_start, *bond_partner;  // CSR by id-1: partner ids (1-based)
+    unsigned char *bond_type;        // per CSR entry, 0-based
+    const void *coef;          // PairCoef<real>[ntypes * ntypes]
+    double *thermo_part;       // [nbrick][kThermoParts]
+    double *thermo_rows;       // [thermo_cap][12]
+    int thermo_cap;
+    RepState *state;
+    // scratch for re-sorting
+    double *scratch;           // 3N doubles
+    int *iscratch;             // 3N ints
+};
+
+// One workgroup's work: replica index and the local chunk/brick index.
+struct WorkItem {
+    int rep;
+    int local;
+};
+
+__host__ __device__ inline int pack_img(int ix, int iy, int iz)
+{
+    return ((ix + 512) & 1023) | (((iy + 512) & 1023) << 10) | (((iz + 512) & 1023) << 20);
+}
+__host__ __device__ inline void unpack_img(int p, int &ix, int &iy, int &iz)
+{
+    ix = (p & 1023) - 512;
+    iy = ((p >> 10) & 1023) - 512;
+    iz = ((p >> 20) & 1023) - 512;
+}
+
+} // namespace gdpd

@@ -1,0 +1,52 @@
+// gdpd_kernels.h -- launch wrappers for the gfx950 kernels in gdpd_kernels.hip.
+#pragma once
+#include "gdpd_internal.h"
+
+namespace gdpd {
+
+struct BrickLaunch {
+    int rmax;      // max region cells ((bx+2)(by+2)(bz+2)) over the launch
+    int smax;      // max staged beads
+    int kmax;      // list entries per owned bead
+    int ncol;      // max inner columns (bx*by)
+    int ntypes;    // max types
+    size_t lds_bytes(int precision, int style) const;
+};
+
+// elementwise over slots (items: rep, chunk of kBlock slots)
+void launch_bin_keys(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s);
+void launch_scatter(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s);
+// cells (items: rep, chunk of kBlock cells)
+void launch_cellsort(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s);
+// scan over cell counts: items of 4*kBlock cells; chunk_off[r] = first chunk of rep r
+void launch_scan(const RepDev *reps, const WorkItem *items, int nitems, const int *rep_chunk_off,
+                 const int *rep_list, int nreps, int *chunk_sums, hipStream_t s);
+void launch_resort(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s);
+// bricks (items: rep, brick)
+void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
+                    int precision, hipStream_t s);
+void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
+                  int precision, int style, int mode, hipStream_t s);
+// per replica (rep_list: nreps replica indices)
+void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
+                     int thermo, hipStream_t s);
+// hook path
+void launch_nve(const RepDev *reps, const WorkItem *items, int nitems, int initial, hipStream_t s);
+void launch_bond_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s);
+void launch_thermo_arrays(const RepDev *reps, const WorkItem *items, int nitems, double *partials,
+                          hipStream_t s);
+// reference neighbour list (single replica)
+void launch_nl_bin(const RepDev *rep, int N, double bin, const int *nb, int *binof, int *count,
+                   hipStream_t s);
+void launch_nl_fill(const RepDev *rep, int N, const int *binof, const int *start, int *fillc,
+                    int *content, hipStream_t s);
+void launch_nl_build(const RepDev *rep, int N, double bin, const int *nb, const int *binof,
+                     const int *start, const int *content, int n_max, int *n_neigh, int *neigh,
+                     int *overflow, double *last_pos, hipStream_t s);
+void launch_nl_check(const RepDev *rep, int N, double skin, const double *last_pos, int *flag,
+                     hipStream_t s);
+// fill helpers (graph-safe replacements for hipMemsetAsync)
+void launch_fill_i32(int *p, int v, long long n, hipStream_t s);
+void launch_fill_f64(double *p, double v, long long n, hipStream_t s);
+
+} // namespace gdpd

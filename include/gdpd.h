@@ -1,0 +1,172 @@
+/*
+ * gdpd.h -- C ABI of the MI355X-native DPD hot path (libgdpd.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes.  No
+ * torch or HIP types appear in any signature.  Host arrays passed in or out are
+ * indexed like the reference's host vectors: bead index = id - 1, positions,
+ * velocities and forces AoS [i*3 + k], virial [i*6 + {xx,xy,xz,yy,yz,zz}]
+ * (system/read_data.cu:265-283, system/system.cuh:6-49).
+ *
+ * Every entry point returns GDPD_OK (0) or a GDPD_ERR_* code; gdpd_last_error()
+ * gives the message.  Nothing calls exit().  A context owns one HIP stream on one
+ * device; calls on one context are not thread-safe.
+ *
+ * Each entry point names the reference interface it replaces (file:line under
+ * the reference's src/).
+ */
+#ifndef GDPD_H
+#define GDPD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDPD_OK 0
+#define GDPD_ERR_ARG 1      /* bad argument (the reference's print_error_and_exit cases) */
+#define GDPD_ERR_HIP 2      /* HIP runtime error */
+#define GDPD_ERR_OVERFLOW 3 /* a capacity bound (staged beads, list length, cell count) was exceeded */
+#define GDPD_ERR_STATE 4    /* call out of order (e.g. step before upload) */
+#define GDPD_ERR_NOMEM 5    /* device allocation or HBM budget refused */
+#define GDPD_ERR_NAN 6      /* non-finite position or velocity detected */
+
+#define GDPD_PREC_F32 0 /* FP32 pair arithmetic, FP64 accumulators and state */
+#define GDPD_PREC_F64 1 /* FP64 everywhere */
+
+#define GDPD_STYLE_NONE 0
+#define GDPD_STYLE_DPD 1
+#define GDPD_STYLE_MDPD 2
+
+/* thermo record layout (compute/thermo.cu:122-214 columns, plus KE and step) */
+#define GDPD_THERMO_T 0
+#define GDPD_THERMO_PAIR_PE 1
+#define GDPD_THERMO_BOND_PE 2
+#define GDPD_THERMO_P 3
+#define GDPD_THERMO_PXX 4 /* 4..9: Pxx Pxy Pxz Pyy Pyz Pzz */
+#define GDPD_THERMO_KE 10
+#define GDPD_THERMO_STEP 11
+#define GDPD_THERMO_LEN 12
+
+typedef struct gdpd_ctx gdpd_ctx;
+
+/* One simulated system ("replica").  Mirrors the reference's System, Box and the
+   `neighbor`, `timestep` and `thermo` commands (main/common.cuh:75-130,
+   main/read_in.cu:93-169, system/system.cuh:53-59). */
+typedef struct {
+    int n_beads;
+    int n_types;
+    double lo[3], hi[3];
+    double dt;
+    double global_cut; /* neighbor global_cut */
+    double skin;       /* neighbor skin */
+    double max_rho;    /* neighbor max_rho: density bound used to size buffers */
+    int check_every;   /* neighbor check_every */
+    int log_every;     /* thermo N */
+    int precision;     /* GDPD_PREC_* */
+} gdpd_system_desc;
+
+/* ---- context -------------------------------------------------------------- */
+/* replaces print_header's cudaSetDevice(0) (main/utils.cuh:52-53) */
+int gdpd_create(gdpd_ctx **out, int device);
+int gdpd_destroy(gdpd_ctx *ctx);
+const char *gdpd_last_error(const gdpd_ctx *ctx);
+/* bytes of device memory a system of this description would take (admission) */
+int64_t gdpd_system_bytes(const gdpd_system_desc *desc, int n_bonds);
+
+/* ---- system state --------------------------------------------------------- */
+/* replaces atoms_mem_alloc + atoms_mem_copy_2_gpu (system/system.cu:5-86) */
+int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *desc, int *sys_out);
+int gdpd_sys_destroy(gdpd_ctx *ctx, int sys);
+/* id[k] in 1..N (any order), type 1-based, pos/vel AoS; positions are wrapped into
+   the box like Read_data::adjust_coordinate (system/read_data.cu:191-197,274-276).
+   masses: n_types values (read_data.cu:114-140). vel may be NULL (zero). */
+int gdpd_upload(gdpd_ctx *ctx, int sys, const int *id, const int *type, const double *pos,
+                const double *vel, const double *masses);
+/* id-ordered copies out; any pointer may be NULL.  image: 3 ints per bead
+   (unwrapped = pos + image * L).  force is the last computed force. */
+int gdpd_download(gdpd_ctx *ctx, int sys, double *pos, double *vel, int *image, double *force);
+/* per-bead pair PE, virial (pair + bond) and mDPD density of the last log step */
+int gdpd_download_pe_virial(gdpd_ctx *ctx, int sys, double *pe, double *virial, double *rho);
+
+/* ---- force fields ---------------------------------------------------------- */
+/* replaces Pair_dpd::isEnabled (force/pair_dpd.cu:19-173): `pair_style dpd T rc seed`
+   + `pair_coeff`.  coeff: 3 doubles {A, gamma, rc} per packed pair index
+   ((ti-1)*nt - (ti-1)(ti-2)/2 + tj - ti, ti <= tj; pair_dpd.cu:134). */
+int gdpd_set_pair_dpd(gdpd_ctx *ctx, int sys, double temp, double rc, uint32_t seed,
+                      const double *coeff);
+/* replaces Pair_mdpd::isEnabled (force/pair_mdpd.cu:21-192): `pair_style mdpd T rc rd
+   seed`; coeff: 5 doubles {A, B, gamma, rc, rd} per packed pair index. */
+int gdpd_set_pair_mdpd(gdpd_ctx *ctx, int sys, double temp, double rc, double rd, uint32_t seed,
+                       const double *coeff);
+/* replaces Bond_harmonic::isEnabled + bonds_mem_copy_2_gpu (force/bond_harmonic.cu:19-147,
+   system/system.cu:123-139): bondlist 3 ints {type, id_i, id_j} per bond;
+   bond_coeff 2 doubles {k, r0} per type, E = k (r - r0)^2. */
+int gdpd_set_bonds(gdpd_ctx *ctx, int sys, int n_bonds, int n_bond_types, const int *bondlist,
+                   const double *bond_coeff);
+
+/* ---- plugin-hook path (one reference hook per call) ----------------------- */
+/* Neigh_list::build (force/neigh_list.cu:397-479): half Verlet list, bins of
+   global_cut + skin, r^2 <= (global_cut+skin)^2, i < j by index. */
+int gdpd_neigh_build(gdpd_ctx *ctx, int sys);
+/* Neigh_list::check_update (force/neigh_list.cu:518-551): *flag = 1 when any bead
+   moved >= skin/2 since the last build. */
+int gdpd_neigh_check(gdpd_ctx *ctx, int sys, int *flag);
+/* copies the last built list out, rows indexed by id-1 and sorted by neighbour
+   index; n_max is the row stride of neigh.  Returns GDPD_ERR_OVERFLOW when a row
+   did not fit the device list (the reference writes out of bounds there,
+   neigh_list.cu:384). */
+int gdpd_neigh_download(gdpd_ctx *ctx, int sys, int *n_neigh, int *neigh, int n_max);
+/* Pair::compute_force (force/pair.cu:77-83 -> pair_dpd.cu:564-633 or
+   pair_mdpd.cu:689-779): zeroes the force (and pe/virial when log) and adds the
+   pair forces at the current positions for Saru step `step`. */
+int gdpd_pair_compute(gdpd_ctx *ctx, int sys, uint32_t step, int log);
+/* Bond::compute_force (force/bond_harmonic.cu:355-402): adds bond forces. */
+int gdpd_bond_compute(gdpd_ctx *ctx, int sys, uint32_t step, int log);
+/* nve::initial_integrate / final_integrate (fix/fix_nve.cu:93-118, 156-177) */
+int gdpd_nve_initial(gdpd_ctx *ctx, int sys);
+int gdpd_nve_final(gdpd_ctx *ctx, int sys);
+/* Thermo::process reductions (compute/thermo.cu:96-220) on the current state:
+   out gets GDPD_THERMO_LEN doubles (step slot = -1). */
+int gdpd_thermo(gdpd_ctx *ctx, int sys, double *out);
+
+/* ---- fused step path -------------------------------------------------------- */
+/* One `run N` (main/utils.cuh:166-314): zero_step at step 0, then steps 1..N, each
+   one fused launch of force (+bonds) + final half-kick + thermo on log steps +
+   next half-kick and drift.  Thermo records of every log step are kept on the
+   device; read them with gdpd_thermo_rows. */
+int gdpd_run(gdpd_ctx *ctx, int sys, int nsteps);
+/* The same split in three for timing: zero_step leaves the system between
+   steps (after step 1's drift); advance runs n whole steps; finish does the
+   last step's force and final half-kick. */
+int gdpd_zero_step(gdpd_ctx *ctx, int sys);
+int gdpd_advance(gdpd_ctx *ctx, int sys, int nsteps);
+int gdpd_finish(gdpd_ctx *ctx, int sys);
+/* thermo records (GDPD_THERMO_LEN doubles each) of the current run, oldest first */
+int gdpd_thermo_rows(gdpd_ctx *ctx, int sys, double *out, int max_rows, int *n_rows);
+/* step counter of the next force evaluation */
+int gdpd_get_step(gdpd_ctx *ctx, int sys, int64_t *step);
+
+/* ---- batched replicas (no reference counterpart: the reference runs one system) */
+/* Advances every listed system by nsteps whole steps, all systems' bricks in the
+   same launches (one launch per phase per step, graph-captured). */
+int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n_systems, int nsteps);
+
+/* ---- tuning and measurement (no reference counterpart) ---------------------- */
+/* brick edge in cells per axis (0 = automatic) and how many steps between
+   physical re-sorts of the bead arrays into cell order (0 = automatic). */
+int gdpd_set_tuning(gdpd_ctx *ctx, int sys, int bx, int by, int bz, int resort_every,
+                    int graph_steps);
+int gdpd_get_tuning(gdpd_ctx *ctx, int sys, int *out8);
+/* Kernel timing: when on, every launch of the fused force kernel is bracketed by
+   HIP events on the context stream; get returns the summed milliseconds and the
+   launch count since the last reset. */
+int gdpd_kernel_timing(gdpd_ctx *ctx, int on);
+int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launches,
+                           double *total_ms);
+int gdpd_sync(gdpd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
