@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Benchmark: bead-steps/s of the fused DPD step on one MI355X.
+
+Workload (default): the 10,077,696-bead DPD fluid (216^3 jittered lattice in a
+150^3 box, rho=3, rc=1, a=25, gamma=4.5, sigma=3, dt=0.04), FP32 pair
+arithmetic with FP64 accumulators and FP64 state.  A "step" is one whole
+velocity-Verlet step: pair force + final half-kick + half-kick + drift + re-binning.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel (the
+fused force+integrate kernel, timed with HIP events on the stream it runs on)
+and a CPU baseline (the OpenMP oracle port on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_BEAD_STEP = {"dpd": 200.0, "mdpd": 244.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds=12.0, n_target=1_000_000):
+    """OpenMP port of the reference force loop (oracle/) on a 1M-bead slice of the
+    same fluid, timed after warm-up; bead-steps/s."""
+    import ctypes as C
+
+    from gpumdpd_amd import configs
+    from oracle import pyoracle
+    cfg = configs.scaled_dpd(n_target, seed=2024)
+    s = pyoracle.make_orc_system(cfg, log_every=10 ** 9)
+    L = pyoracle.lib()
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    h = L.orc_omp_create(C.byref(s), nthreads)
+    threads = L.orc_omp_threads(h)
+    L.orc_omp_steps(h, 0, 2)  # warm-up: step 0 + one step
+    steps, t0, step = 0, time.perf_counter(), 2
+    while True:
+        L.orc_omp_steps(h, step, 1)
+        step += 1
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > seconds or steps >= 200:
+            break
+    L.orc_omp_destroy(h)
+    return {"value": cfg["N"] * steps / el, "unit": "bead-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{cfg['N']}-bead DPD fluid (rho=3, same force field), {steps} steps after 2 warm-up, "
+                      f"half neighbour list rebuilt on the skin criterion, {threads} OpenMP threads"}
+
+
+def load_traffic(workload):
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(workload)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c2", "c3"])
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(this build runs a single system or a batch of replicas on one MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    import gpumdpd_amd as g
+    from gpumdpd_amd import configs
+    t = time.perf_counter()
+    cfg = {"c0": configs.config_c0, "c1": configs.config_c1, "c2": configs.config_c2,
+           "c3": configs.config_c3}[args.config]()
+    log(f"bench: generated {cfg['name']} in {time.perf_counter() - t:.1f}s")
+    prec = g.PREC_F32 if args.precision == "f32" else g.PREC_F64
+    ctx = g.Context(0)
+    s = g.system_from_config(ctx, cfg, precision=prec, log_every=10 ** 6)
+    tun = s.tuning()
+    log(f"bench: uploaded N={cfg['N']} tuning={tun}")
+    s.zero_step()
+    s.advance(args.warmup)
+    ctx.sync()
+    log(f"bench: warm-up {args.warmup} steps done")
+
+    ctx.kernel_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.advance(args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    k_ms, k_n, _ = ctx.kernel_timing_get()
+    ctx.kernel_timing(False)
+    s.finish()
+    log(f"bench: timed {args.steps} steps in {wall:.3f}s")
+
+    N = cfg["N"]
+    value = N * args.steps / wall
+    per = BYTES_PER_BEAD_STEP[cfg["style"]] + (12.0 * len(cfg["bonds"]) / N if "bonds" in cfg else 0.0)
+    k_avg = k_ms / max(k_n, 1) / 1e3
+    achieved = per * N / k_avg / 1e9
+    workload = {"c0": "dpd_fluid_24k", "c1": "mdpd_slab_8k", "c2": "diblock_melt_375k",
+                "c3": "dpd_fluid_10M"}[args.config]
+    out = {
+        "metric": "bead-steps/s at 10M-bead DPD (FP32 force)" if args.config == "c3" else f"bead-steps/s ({workload})",
+        "value": value,
+        "unit": "bead-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == g.PREC_F32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "n_beads": N, "box": list(map(float, cfg["hi"])), "rho": 3.0,
+                   "precision": "fp32 pair force, fp64 accumulators and state" if prec == g.PREC_F32 else "fp64",
+                   "brick": [tun["bx"], tun["by"], tun["bz"]], "resort_every": tun["resort_every"],
+                   "graph_steps": 0},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
+                     "kernel": "k_brick (fused pair force + integrate)", "kernel_ms": k_avg * 1e3,
+                     "bytes_per_bead_step": per},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        log("bench: CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -643,7 +643,7 @@ int gdpd_create(gdpd_ctx **out, int device)
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
-        ctx->max_lds = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 * 1024);
+        ctx->max_lds = (int)std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
     *out = ctx;
     return GDPD_OK;
 }
@@ -1347,6 +1347,34 @@ int gdpd_sync(gdpd_ctx *ctx)
 {
     if (!ctx) return GDPD_ERR_ARG;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+int gdpd_saru_eval(gdpd_ctx *ctx, int n, const uint32_t *tag1, const uint32_t *tag2, uint32_t seed,
+                   const int32_t *steps, uint32_t *bits, float *r32, double *r64)
+{
+    if (!ctx || n < 0 || (n > 0 && (!tag1 || !tag2 || !steps))) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (n == 0) return GDPD_OK;
+    char *buf = nullptr;
+    const size_t b4 = sizeof(uint32_t) * (size_t)n;
+    HIPCHK(hipMalloc((void **)&buf, b4 * 6 + sizeof(double) * (size_t)n));
+    uint32_t *d1 = (uint32_t *)buf, *d2 = d1 + n, *db = d2 + n;
+    int32_t *ds = (int32_t *)(db + n);
+    float *dr32 = (float *)(ds + n);
+    double *dr64 = (double *)(buf + b4 * 6);
+    hipError_t e = hipMemcpyAsync(d1, tag1, b4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d2, tag2, b4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, steps, b4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        launch_saru_eval(n, d1, d2, seed, ds, db, dr32, dr64, ctx->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && bits) e = hipMemcpyAsync(bits, db, b4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && r32) e = hipMemcpyAsync(r32, dr32, b4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && r64) e = hipMemcpyAsync(r64, dr64, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(ctx, GDPD_ERR_HIP, "saru eval failed: %s", hipGetErrorString(e));
     return GDPD_OK;
 }
 

@@ -45,6 +45,8 @@ void launch_nl_build(const RepDev *rep, int N, double bin, const int *nb, const 
                      int *overflow, double *last_pos, hipStream_t s);
 void launch_nl_check(const RepDev *rep, int N, double skin, const double *last_pos, int *flag,
                      hipStream_t s);
+void launch_saru_eval(int n, const uint32_t *t1, const uint32_t *t2, uint32_t seed, const int32_t *steps,
+                      uint32_t *bits, float *r32, double *r64, hipStream_t s);
 // fill helpers (graph-safe replacements for hipMemsetAsync)
 void launch_fill_i32(int *p, int v, long long n, hipStream_t s);
 void launch_fill_f64(double *p, double v, long long n, hipStream_t s);

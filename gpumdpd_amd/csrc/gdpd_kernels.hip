@@ -1016,6 +1016,20 @@ __global__ __launch_bounds__(kBlock) void k_nl_check(const RepDev *__restrict__ 
     if (dx * dx + dy * dy + dz * dz >= hs2) *flag = 1;
 }
 
+__global__ __launch_bounds__(kBlock) void k_saru_eval(int n, const uint32_t *__restrict__ t1,
+                                                      const uint32_t *__restrict__ t2, uint32_t seed,
+                                                      const int32_t *__restrict__ steps,
+                                                      uint32_t *__restrict__ bits, float *__restrict__ r32,
+                                                      double *__restrict__ r64)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t b = saru_bits(t1[k], t2[k], seed + (uint32_t)steps[k]);
+    bits[k] = b;
+    r32[k] = saru_uniform<float>(b);
+    r64[k] = saru_uniform<double>(b);
+}
+
 __global__ void k_fill_i32(int *p, int v, long long n)
 {
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -1141,6 +1155,11 @@ void launch_nl_check(const RepDev *rep, int N, double skin, const double *last_p
 {
     const double hs2 = (skin * 0.5) * (skin * 0.5);
     if (N) hipLaunchKernelGGL(k_nl_check, dim3(nblk(N)), dim3(kBlock), 0, s, rep, hs2, last_pos, flag);
+}
+void launch_saru_eval(int n, const uint32_t *t1, const uint32_t *t2, uint32_t seed, const int32_t *steps,
+                      uint32_t *bits, float *r32, double *r64, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_saru_eval, dim3(nblk(n)), dim3(kBlock), 0, s, n, t1, t2, seed, steps, bits, r32, r64);
 }
 void launch_fill_i32(int *p, int v, long long n, hipStream_t s)
 {

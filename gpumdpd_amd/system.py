@@ -1,0 +1,221 @@
+"""Python front of libgdpd: a device context and the systems ("replicas") on it.
+
+Method names follow the reference's plugin hooks so tests read like the
+reference's step loop (main/utils.cuh:166-314):
+    Pair::compute_force        -> System.pair_compute(step, log)
+    Bond::compute_force        -> System.bond_compute(step, log)
+    nve::initial_integrate     -> System.nve_initial()
+    nve::final_integrate       -> System.nve_final()
+    Neigh_list::build          -> System.neigh_build()
+    Neigh_list::check_update   -> System.neigh_check()
+    Thermo::process            -> System.thermo()
+and the fused path runs a whole `run N` (System.run) in one launch per step.
+
+Host arrays are indexed like the reference's host vectors: row = id - 1,
+positions/velocities/forces (N, 3), virial (N, 6).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+class Context:
+    """One device + one HIP stream (libgdpd gdpd_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.gdpd_create(C.byref(h), device)
+        if rc != L.GDPD_OK:
+            raise L.GdpdError(rc, f"gdpd_create(device={device}) failed")
+        self.h = h
+
+    def check(self, rc):
+        if rc != L.GDPD_OK:
+            raise L.GdpdError(rc, self.lib.gdpd_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.gdpd_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        self.check(self.lib.gdpd_sync(self.h))
+
+    def batch_advance(self, systems, nsteps):
+        ids = np.array([s.sid for s in systems], dtype=np.int32)
+        self.check(self.lib.gdpd_batch_advance(self.h, _ptr(ids, C.c_int), len(ids), int(nsteps)))
+
+    def saru_eval(self, tag1, tag2, seed, steps):
+        t1 = np.ascontiguousarray(tag1, dtype=np.uint32)
+        t2 = np.ascontiguousarray(tag2, dtype=np.uint32)
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        n = len(t1)
+        bits, r32, r64 = np.empty(n, np.uint32), np.empty(n, np.float32), np.empty(n, np.float64)
+        self.check(self.lib.gdpd_saru_eval(self.h, n, _ptr(t1, C.c_uint32), _ptr(t2, C.c_uint32),
+                                           int(seed) & 0xFFFFFFFF, _ptr(st, C.c_int32), _ptr(bits, C.c_uint32),
+                                           _ptr(r32, C.c_float), _ptr(r64, C.c_double)))
+        return bits, r32, r64
+
+    def kernel_timing(self, on):
+        self.check(self.lib.gdpd_kernel_timing(self.h, 1 if on else 0))
+
+    def kernel_timing_get(self):
+        ms, n, tot = C.c_double(), C.c_int64(), C.c_double()
+        self.check(self.lib.gdpd_kernel_timing_get(self.h, C.byref(ms), C.byref(n), C.byref(tot)))
+        return ms.value, n.value, tot.value
+
+
+def make_desc(n_beads, n_types, lo, hi, dt, global_cut=1.0, skin=0.3, max_rho=6.0,
+              check_every=1, log_every=100, precision=L.PREC_F32):
+    d = L.SystemDesc()
+    d.n_beads, d.n_types = int(n_beads), int(n_types)
+    for k in range(3):
+        d.lo[k], d.hi[k] = float(lo[k]), float(hi[k])
+    d.dt, d.global_cut, d.skin, d.max_rho = float(dt), float(global_cut), float(skin), float(max_rho)
+    d.check_every, d.log_every, d.precision = int(check_every), int(log_every), int(precision)
+    return d
+
+
+def system_bytes(desc, n_bonds=0):
+    return int(L.load().gdpd_system_bytes(C.byref(desc), int(n_bonds)))
+
+
+class System:
+    """One simulated system on a Context (libgdpd system handle)."""
+
+    def __init__(self, ctx, desc):
+        self.ctx, self.lib, self.desc = ctx, ctx.lib, desc
+        sid = C.c_int()
+        ctx.check(self.lib.gdpd_sys_create(ctx.h, C.byref(desc), C.byref(sid)))
+        self.sid = sid.value
+        self.N = desc.n_beads
+
+    def destroy(self):
+        if self.sid is not None and self.ctx.h:
+            self.ctx.check(self.lib.gdpd_sys_destroy(self.ctx.h, self.sid))
+        self.sid = None
+
+    # ---- state ----
+    def upload(self, ids, types, pos, vel, masses):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        types = np.ascontiguousarray(types, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        masses = np.ascontiguousarray(masses, dtype=np.float64)
+        self.ctx.check(self.lib.gdpd_upload(self.ctx.h, self.sid, _ptr(ids, C.c_int), _ptr(types, C.c_int),
+                                            _ptr(pos, C.c_double), _ptr(vel, C.c_double),
+                                            _ptr(masses, C.c_double)))
+
+    def download(self):
+        N = self.N
+        pos, vel, force = (np.empty((N, 3)) for _ in range(3))
+        img = np.empty((N, 3), dtype=np.int32)
+        self.ctx.check(self.lib.gdpd_download(self.ctx.h, self.sid, _ptr(pos, C.c_double),
+                                              _ptr(vel, C.c_double), _ptr(img, C.c_int),
+                                              _ptr(force, C.c_double)))
+        return {"pos": pos, "vel": vel, "image": img, "force": force}
+
+    def download_pe_virial(self):
+        N = self.N
+        pe, vir, rho = np.empty(N), np.empty((N, 6)), np.empty(N)
+        self.ctx.check(self.lib.gdpd_download_pe_virial(self.ctx.h, self.sid, _ptr(pe, C.c_double),
+                                                        _ptr(vir, C.c_double), _ptr(rho, C.c_double)))
+        return pe, vir, rho
+
+    # ---- force fields ----
+    def set_pair_dpd(self, temp, rc, seed, coeff):
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        self.ctx.check(self.lib.gdpd_set_pair_dpd(self.ctx.h, self.sid, float(temp), float(rc),
+                                                  int(seed) & 0xffffffff, _ptr(c, C.c_double)))
+
+    def set_pair_mdpd(self, temp, rc, rd, seed, coeff):
+        c = np.ascontiguousarray(coeff, dtype=np.float64)
+        self.ctx.check(self.lib.gdpd_set_pair_mdpd(self.ctx.h, self.sid, float(temp), float(rc), float(rd),
+                                                   int(seed) & 0xffffffff, _ptr(c, C.c_double)))
+
+    def set_bonds(self, bondlist, n_bond_types, bond_coeff):
+        b = np.ascontiguousarray(bondlist, dtype=np.int32).reshape(-1, 3)
+        c = np.ascontiguousarray(bond_coeff, dtype=np.float64)
+        self.ctx.check(self.lib.gdpd_set_bonds(self.ctx.h, self.sid, len(b), int(n_bond_types),
+                                               _ptr(b, C.c_int), _ptr(c, C.c_double)))
+
+    # ---- hooks ----
+    def neigh_build(self):
+        self.ctx.check(self.lib.gdpd_neigh_build(self.ctx.h, self.sid))
+
+    def neigh_check(self):
+        f = C.c_int()
+        self.ctx.check(self.lib.gdpd_neigh_check(self.ctx.h, self.sid, C.byref(f)))
+        return f.value
+
+    def neigh_download(self, n_max):
+        n = np.empty(self.N, dtype=np.int32)
+        nb = np.full((self.N, n_max), -1, dtype=np.int32)
+        self.ctx.check(self.lib.gdpd_neigh_download(self.ctx.h, self.sid, _ptr(n, C.c_int),
+                                                    _ptr(nb, C.c_int), int(n_max)))
+        return n, nb
+
+    def pair_compute(self, step, log=True):
+        self.ctx.check(self.lib.gdpd_pair_compute(self.ctx.h, self.sid, int(step), 1 if log else 0))
+
+    def bond_compute(self, step, log=True):
+        self.ctx.check(self.lib.gdpd_bond_compute(self.ctx.h, self.sid, int(step), 1 if log else 0))
+
+    def nve_initial(self):
+        self.ctx.check(self.lib.gdpd_nve_initial(self.ctx.h, self.sid))
+
+    def nve_final(self):
+        self.ctx.check(self.lib.gdpd_nve_final(self.ctx.h, self.sid))
+
+    def thermo(self):
+        out = np.empty(L.THERMO_LEN)
+        self.ctx.check(self.lib.gdpd_thermo(self.ctx.h, self.sid, _ptr(out, C.c_double)))
+        return out
+
+    # ---- fused path ----
+    def run(self, nsteps):
+        self.ctx.check(self.lib.gdpd_run(self.ctx.h, self.sid, int(nsteps)))
+
+    def zero_step(self):
+        self.ctx.check(self.lib.gdpd_zero_step(self.ctx.h, self.sid))
+
+    def advance(self, nsteps):
+        self.ctx.check(self.lib.gdpd_advance(self.ctx.h, self.sid, int(nsteps)))
+
+    def finish(self):
+        self.ctx.check(self.lib.gdpd_finish(self.ctx.h, self.sid))
+
+    def thermo_rows(self, max_rows=4096):
+        out = np.empty((max_rows, L.THERMO_LEN))
+        n = C.c_int()
+        self.ctx.check(self.lib.gdpd_thermo_rows(self.ctx.h, self.sid, _ptr(out, C.c_double), max_rows,
+                                                 C.byref(n)))
+        return out[: n.value].copy()
+
+    def step(self):
+        s = C.c_int64()
+        self.ctx.check(self.lib.gdpd_get_step(self.ctx.h, self.sid, C.byref(s)))
+        return s.value
+
+    def set_tuning(self, brick=(0, 0, 0), resort_every=-1, graph_steps=-1):
+        self.ctx.check(self.lib.gdpd_set_tuning(self.ctx.h, self.sid, int(brick[0]), int(brick[1]),
+                                                int(brick[2]), int(resort_every), int(graph_steps)))
+
+    def tuning(self):
+        out = np.zeros(8, dtype=np.int32)
+        self.ctx.check(self.lib.gdpd_get_tuning(self.ctx.h, self.sid, _ptr(out, C.c_int)))
+        keys = ["bx", "by", "bz", "smax", "kmax", "resort_every", "graph_steps", "nbrick"]
+        return dict(zip(keys, out.tolist()))

@@ -166,6 +166,13 @@ int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launc
                            double *total_ms);
 int gdpd_sync(gdpd_ctx *ctx);
 
+/* Evaluates the device Saru hash (the pair-force kernels' own inline function)
+   for n host tuples: bits, the FP32-build uniform and the FP64-build uniform.
+   Replaces nothing (the reference's saru() is device-inline, force/pair_dpd.cu:212);
+   exposed so the hash can be checked bit for bit.  Any output may be NULL. */
+int gdpd_saru_eval(gdpd_ctx *ctx, int n, const uint32_t *tag1, const uint32_t *tag2, uint32_t seed,
+                   const int32_t *steps, uint32_t *bits, float *r32, double *r64);
+
 #ifdef __cplusplus
 }
 #endif

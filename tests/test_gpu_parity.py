@@ -1,0 +1,115 @@
+"""GPU parity of the HIP path against the FP64 oracle, through the C ABI.
+
+Tolerances (from the build's precision contract):
+  FP32 pair arithmetic / FP64 accumulation: global relative L2 force error <= 1e-5
+  FP64 mode: <= 1e-11
+"""
+import numpy as np
+import pytest
+
+import gpumdpd_amd as g
+from gpumdpd_amd import configs
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+F32_TOL = 1e-5
+F64_TOL = 1e-11
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = g.Context(0)
+    yield c
+    c.close()
+
+
+def test_gpu_saru_bits_match_golden(ctx):
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "saru_table.npz"))
+    for sd in np.unique(d["seed"]):
+        m = d["seed"] == sd
+        bits, r32, r64 = ctx.saru_eval(d["tag1"][m], d["tag2"][m], int(sd), d["step"][m])
+        assert np.array_equal(bits, d["bits"][m])
+        assert np.array_equal(r32.view(np.uint32), d["r32"][m].view(np.uint32))
+        assert np.array_equal(r64.view(np.uint64), d["r64"][m].view(np.uint64))
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)])
+@pytest.mark.parametrize("step", [0, 17])
+def test_dpd_force_c0(ctx, prec, tol, step):
+    cfg = configs.config_c0()
+    print(f"case: C0 prec={prec} step={step}", flush=True)
+    s = g.system_from_config(ctx, cfg, precision=prec)
+    s.pair_compute(step=step, log=True)
+    f = s.download()["force"]
+    pe, vir, _ = s.download_pe_virial()
+    f_ref, pe_ref, _, vir_ref, _ = pyoracle.forces(cfg, step=step)
+    assert rel(f, f_ref) < tol
+    assert rel(pe, pe_ref) < tol
+    assert rel(vir, vir_ref) < tol
+    s.destroy()
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)])
+def test_mdpd_density_and_force_c1(ctx, prec, tol):
+    cfg = configs.config_c1()
+    print(f"case: C1 prec={prec}", flush=True)
+    s = g.system_from_config(ctx, cfg, precision=prec)
+    s.pair_compute(step=3, log=True)
+    f = s.download()["force"]
+    pe, vir, rho = s.download_pe_virial()
+    f_ref, pe_ref, _, vir_ref, rho_ref = pyoracle.forces(cfg, step=3)
+    assert rel(rho, rho_ref) < tol
+    assert rel(f, f_ref) < tol
+    assert rel(pe, pe_ref) < tol
+    s.destroy()
+
+
+def test_bonded_melt_slice(ctx):
+    cfg = configs.config_c2(n_chains=1500, box=20.0)
+    print("case: C2 slice", flush=True)
+    for prec, tol in [(g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)]:
+        s = g.system_from_config(ctx, cfg, precision=prec)
+        s.pair_compute(step=5, log=True)
+        s.bond_compute(step=5, log=True)
+        f = s.download()["force"]
+        f_ref, _, _, _, _ = pyoracle.forces(cfg, step=5)
+        assert rel(f, f_ref) < tol
+        s.destroy()
+
+
+def test_fused_run_matches_oracle_fp64(ctx):
+    cfg = configs.scaled_dpd(4000, seed=3)
+    print("case: fused 30-step FP64 run", flush=True)
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=10)
+    s.run(30)
+    st = s.download()
+    rows = s.thermo_rows()
+    ref_rows, ref_state, _ = pyoracle.run(cfg, 30, log_every=10)
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    d = st["pos"] - ref_state["pos"]
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-9
+    assert np.max(np.abs(st["vel"] - ref_state["vel"])) < 1e-8
+    assert rel(st["force"], ref_state["force"]) < 1e-9
+    assert len(rows) == len(ref_rows) == 4
+    assert np.allclose(rows[:, :11], ref_rows, rtol=1e-9, atol=1e-9)
+    s.destroy()
+
+
+def test_reference_neighbour_list(ctx):
+    cfg = configs.scaled_dpd(3000, seed=5)
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F64)
+    s.neigh_build()
+    n_max = 96
+    n, nb = s.neigh_download(n_max)
+    on, onb = pyoracle.neigh_build(cfg["pos"], cfg["lo"], cfg["hi"], 1.3, n_max)
+    assert np.array_equal(n, on)
+    assert np.array_equal(np.where(nb >= 0, nb, -1), np.where(onb >= 0, onb, -1))
+    assert s.neigh_check() == 0
+    s.destroy()
