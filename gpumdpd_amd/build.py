@@ -27,6 +27,23 @@ def hipcc_command(out=LIB):
     ] + srcs
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "plugin_driver.cpp")
+DRIVER_BIN = os.path.join(ROOT, "tests", "cpp", "plugin_driver")
+
+
+def build_driver(verbose=False):
+    """The C++ plugin-shell driver used by the GPU tests (links libgdpd.so)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", DRIVER_BIN,
+           "-L", HERE, "-lgdpd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../gpumdpd_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the plugin driver")
+    return DRIVER_BIN
+
+
 def build(verbose=False):
     cmd = hipcc_command(LIB + ".tmp")
     if verbose:
@@ -36,6 +53,8 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libgdpd.so")
     os.replace(LIB + ".tmp", LIB)
+    if os.path.exists(DRIVER_SRC):
+        build_driver(verbose)
     return LIB
 
 

@@ -83,7 +83,7 @@ __device__ __forceinline__ int wave_incl_scan(int x)
 }
 
 // exclusive scan over the 256 threads; every thread must call it
-__device__ int block_excl_scan(int v, int *wbuf, int &total)
+__device__ __forceinline__ int block_excl_scan(int v, int *wbuf, int &total)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = wave_incl_scan(v);
@@ -110,7 +110,7 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // fixed-order block sum of n doubles per thread into out[0..n) of thread 0
 template <int NV>
-__device__ void block_sum(double (&acc)[NV], double *red, double *out)
+__device__ __forceinline__ void block_sum(double (&acc)[NV], double *red, double *out)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -222,8 +222,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RepDev *__restrict__ r
     else atomicOr(&R.state->flags, kFlagStageOverflow);
 }
 
-// sort each cell's slots ascending so the in-cell order (and with it every
-// per-bead summation order) is independent of atomic scheduling
+// sort each cell's beads by id so the in-cell order, and with it every
+// per-bead summation order, depends only on the state: not on atomic
+// scheduling, on the slot layout (re-sort interval) or on the batch
 __global__ __launch_bounds__(kBlock) void k_cellsort(const RepDev *__restrict__ reps,
                                                      const WorkItem *__restrict__ items, int nitems)
 {
@@ -233,10 +234,11 @@ __global__ __launch_bounds__(kBlock) void k_cellsort(const RepDev *__restrict__ 
     if (c >= R.ncell) return;
     const int a = R.cell_start[c], b = R.cell_start[c + 1];
     int *p = R.perm;
+    const int *id = R.id;
     for (int i = a + 1; i < b; ++i) {
-        const int v = p[i];
+        const int v = p[i], kv = id[v];
         int j = i - 1;
-        while (j >= a && p[j] > v) {
+        while (j >= a && id[p[j]] > kv) {
             p[j + 1] = p[j];
             --j;
         }
@@ -303,34 +305,66 @@ template <typename real> struct BrickLds {
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
+// LDS layout of the brick kernels as byte offsets.  The pointers are rebuilt
+// inline from the __shared__ base in every kernel so they stay in the LDS
+// address space (ds_read/ds_write), never in generic pointers or scratch.
+struct LdsOffsets {
+    unsigned coef, red, px, py, pz, vx, vy, vz, rh, c_start, c_src, sid, colpre, wbuf, scell, list, sty,
+        total;
+};
+
 template <typename real>
-__host__ __device__ size_t brick_lds_carve(const BrickParams &p, char *base, BrickLds<real> *L)
+__host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
 {
+    LdsOffsets o;
     size_t off = 0;
-    auto take = [&](size_t bytes) {
-        char *q = base + off;
-        off = align16(off + bytes);
-        return q;
-    };
-    char *c;
-    c = take(sizeof(PairCoef<real>) * p.ntypes * p.ntypes); if (L) L->coef = (PairCoef<real> *)c;
-    c = take(sizeof(double) * 64);                          if (L) L->red = (double *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->px = (real *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->py = (real *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->pz = (real *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->vx = (real *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->vy = (real *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->vz = (real *)c;
-    c = take(sizeof(real) * p.smax);                        if (L) L->rh = (real *)c;
-    c = take(sizeof(int) * (p.rmax + 1));                   if (L) L->c_start = (int *)c;
-    c = take(sizeof(int) * p.rmax);                         if (L) L->c_src = (int *)c;
-    c = take(sizeof(int) * p.smax);                         if (L) L->sid = (int *)c;
-    c = take(sizeof(int) * (p.ncol + 1));                   if (L) L->colpre = (int *)c;
-    c = take(sizeof(int) * (kBlock / 64));                  if (L) L->wbuf = (int *)c;
-    c = take(sizeof(unsigned short) * p.smax);              if (L) L->scell = (unsigned short *)c;
-    c = take(sizeof(unsigned short) * (size_t)p.kmax * kBlock); if (L) L->list = (unsigned short *)c;
-    c = take(sizeof(unsigned char) * p.smax);               if (L) L->sty = (unsigned char *)c;
-    return off;
+#define GDPD_TAKE(field, bytes)   \
+    o.field = (unsigned)off;      \
+    off = align16(off + (size_t)(bytes));
+    GDPD_TAKE(coef, sizeof(PairCoef<real>) * p.ntypes * p.ntypes)
+    GDPD_TAKE(red, sizeof(double) * 64)
+    GDPD_TAKE(px, sizeof(real) * p.smax)
+    GDPD_TAKE(py, sizeof(real) * p.smax)
+    GDPD_TAKE(pz, sizeof(real) * p.smax)
+    GDPD_TAKE(vx, sizeof(real) * p.smax)
+    GDPD_TAKE(vy, sizeof(real) * p.smax)
+    GDPD_TAKE(vz, sizeof(real) * p.smax)
+    GDPD_TAKE(rh, sizeof(real) * p.smax)
+    GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
+    GDPD_TAKE(c_src, sizeof(int) * p.rmax)
+    GDPD_TAKE(sid, sizeof(int) * p.smax)
+    GDPD_TAKE(colpre, sizeof(int) * (p.ncol + 1))
+    GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
+    GDPD_TAKE(scell, sizeof(unsigned short) * p.smax)
+    GDPD_TAKE(list, sizeof(unsigned short) * (size_t)p.kmax * kBlock)
+    GDPD_TAKE(sty, sizeof(unsigned char) * p.smax)
+#undef GDPD_TAKE
+    o.total = (unsigned)off;
+    return o;
+}
+
+template <typename real>
+__device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets &o)
+{
+    BrickLds<real> L;
+    L.coef = (PairCoef<real> *)(smem + o.coef);
+    L.red = (double *)(smem + o.red);
+    L.px = (real *)(smem + o.px);
+    L.py = (real *)(smem + o.py);
+    L.pz = (real *)(smem + o.pz);
+    L.vx = (real *)(smem + o.vx);
+    L.vy = (real *)(smem + o.vy);
+    L.vz = (real *)(smem + o.vz);
+    L.rh = (real *)(smem + o.rh);
+    L.c_start = (int *)(smem + o.c_start);
+    L.c_src = (int *)(smem + o.c_src);
+    L.sid = (int *)(smem + o.sid);
+    L.colpre = (int *)(smem + o.colpre);
+    L.wbuf = (int *)(smem + o.wbuf);
+    L.scell = (unsigned short *)(smem + o.scell);
+    L.list = (unsigned short *)(smem + o.list);
+    L.sty = (unsigned char *)(smem + o.sty);
+    return L;
 }
 
 struct BrickGeom {
@@ -373,7 +407,7 @@ __device__ __forceinline__ int wrap_cell(int c, int n, double L, double &shift)
 // Stage the brick + halo into LDS.  Returns the number of staged beads, or -1
 // (uniformly) when it does not fit.  need: 1 = velocities + ids, 2 = density.
 template <typename real>
-__device__ int stage_brick(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
+__device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
                            const BrickParams &bp, int cur, bool need_vel, bool need_rho)
 {
     const int t = threadIdx.x;
@@ -483,8 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
                                                     BrickParams bp)
 {
     extern __shared__ __align__(16) char smem[];
-    BrickLds<real> L;
-    brick_lds_carve<real>(bp, smem, &L);
+    const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int cur = R.state->cur;
@@ -539,8 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
                                                   BrickParams bp, int mode)
 {
     extern __shared__ __align__(16) char smem[];
-    BrickLds<real> L;
-    brick_lds_carve<real>(bp, smem, &L);
+    const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int cur = R.state->cur;
@@ -555,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         return;
     }
     const int t = threadIdx.x;
-    const int N = R.N, nt = R.ntypes, kmax = bp.kmax;
+    const int N = R.N, nt = R.ntypes, kmax = max(bp.kmax, 1);
     const int O = L.colpre[g.ext[0] * g.ext[1]];
     const real rcut2 = (real)R.rcmax2;
     const uint32_t seed_step = R.seed + (uint32_t)step;
@@ -571,79 +603,87 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         const int rc = L.scell[li], rz = rc % g.RZ;
         const real xi = L.px[li], yi = L.py[li], zi = L.pz[li];
 
-        // phase 1: in-range list of staged neighbours (27 cells = 9 z-runs)
-        int cnt = 0;
-        for (int ox = -1; ox <= 1; ++ox)
-            for (int oy = -1; oy <= 1; ++oy) {
-                const int base = ((rx + ox) * g.RY + (ry + oy)) * g.RZ + rz - 1;
-                const int j1 = L.c_start[base + 3];
-                for (int lj = L.c_start[base]; lj < j1; ++lj) {
-                    const real dx = xi - L.px[lj], dy = yi - L.py[lj], dz = zi - L.pz[lj];
-                    const real r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 <= rcut2 && r2 >= (real)1.0e-20 && lj != li) {
-                        if (cnt < kmax) L.list[cnt * kBlock + t] = (unsigned short)lj;
-                        ++cnt;
-                    }
-                }
-            }
-        if (cnt > kmax) {
-            atomicOr(&R.state->flags, kFlagListOverflow);
-            cnt = kmax;
-        }
-
-        // phase 2: pair forces
         const int idi = L.sid[li];
         const int ti = L.sty[li] - 1;
         const real vxi = L.vx[li], vyi = L.vy[li], vzi = L.vz[li];
         const real rhoi = (STYLE == 2) ? L.rh[li] : (real)0;
         double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
         double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
-        for (int k = 0; k < cnt; ++k) {
-            const int lj = L.list[k * kBlock + t];
-            const real dx = xi - L.px[lj], dy = yi - L.py[lj], dz = zi - L.pz[lj];
-            const real r2 = dx * dx + dy * dy + dz * dz;
-            const PairCoef<real> &C = L.coef[ti * nt + (L.sty[lj] - 1)];
-            if (r2 > C.rc2) continue;
-            const real r = sqrt(r2);
-            const real rinv = (real)1 / r;
-            const real dot = dx * (vxi - L.vx[lj]) + dy * (vyi - L.vy[lj]) + dz * (vzi - L.vz[lj]);
-            const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
-            const real wr = wc;
-            const int idj = L.sid[lj];
-            const uint32_t t1 = (uint32_t)min(idi, idj), t2 = (uint32_t)max(idi, idj);
-            const real rnd = saru_uniform<real>(saru_bits(t1, t2, seed_step));
-            real wd = (real)0;
-            real fpair;
-            if (STYLE == 2) {
-                wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
-                wd = wd > (real)0 ? wd : (real)0;
-                fpair = C.A * wc + C.B * (rhoi + L.rh[lj]) * wd;
-            } else {
-                fpair = C.A * wc;
-            }
-            fpair -= C.gamma * wr * wr * dot * rinv;
-            fpair += C.sig * wr * rnd;
-            fpair *= rinv;
-            const real ffx = dx * fpair, ffy = dy * fpair, ffz = dz * fpair;
-            fx += (double)ffx;
-            fy += (double)ffy;
-            fz += (double)ffz;
-            if (log) {
-                if (STYLE == 2) {
-                    const real u_c = (real)0.5 * C.A * C.rc * wc * wc;
-                    const real u_d = (real)0.25 * C.B * C.rd * rhoi * wd * wd;
-                    pe += (double)((real)0.5 * u_c + u_d);
-                } else {
-                    const real ppe = (real)0.5 * C.A * C.rc * wc * wc;
-                    pe += (double)((real)0.5 * ppe);
+        // The 27 neighbour cells are 9 z-runs of 3 cells, contiguous in LDS.
+        // Phase 1 appends in-range neighbours to this thread's list; when the
+        // list is full, phase 2 evaluates it and phase 1 resumes at its cursor,
+        // so the summation order is the candidate order whatever kmax is.
+        int run = 0, lj = 0, j1 = 0;
+        {
+            const int base = ((rx - 1) * g.RY + (ry - 1)) * g.RZ + rz - 1;
+            lj = L.c_start[base];
+            j1 = L.c_start[base + 3];
+        }
+        while (true) {
+            int cnt = 0;
+            while (run < 9) {
+                if (lj >= j1) {
+                    if (++run == 9) break;
+                    const int base = ((rx - 1 + run / 3) * g.RY + (ry - 1 + run % 3)) * g.RZ + rz - 1;
+                    lj = L.c_start[base];
+                    j1 = L.c_start[base + 3];
+                    continue;
                 }
-                w0 += (double)((real)0.5 * dx * ffx);
-                w1 += (double)((real)0.5 * dx * ffy);
-                w2 += (double)((real)0.5 * dx * ffz);
-                w3 += (double)((real)0.5 * dy * ffy);
-                w4 += (double)((real)0.5 * dy * ffz);
-                w5 += (double)((real)0.5 * dz * ffz);
+                if (cnt == kmax) break;
+                const real dx = xi - L.px[lj], dy = yi - L.py[lj], dz = zi - L.pz[lj];
+                const real r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= rcut2 && r2 >= (real)1.0e-20 && lj != li) L.list[(cnt++) * kBlock + t] = (unsigned short)lj;
+                ++lj;
             }
+
+            for (int k = 0; k < cnt; ++k) {
+                const int lk = L.list[k * kBlock + t];
+                const real dx = xi - L.px[lk], dy = yi - L.py[lk], dz = zi - L.pz[lk];
+                const real r2 = dx * dx + dy * dy + dz * dz;
+                const PairCoef<real> &C = L.coef[ti * nt + (L.sty[lk] - 1)];
+                if (r2 > C.rc2) continue;
+                const real r = sqrt(r2);
+                const real rinv = (real)1 / r;
+                const real dot = dx * (vxi - L.vx[lk]) + dy * (vyi - L.vy[lk]) + dz * (vzi - L.vz[lk]);
+                const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
+                const real wr = wc;
+                const int idj = L.sid[lk];
+                const uint32_t t1 = (uint32_t)min(idi, idj), t2 = (uint32_t)max(idi, idj);
+                const real rnd = saru_uniform<real>(saru_bits(t1, t2, seed_step));
+                real wd = (real)0;
+                real fpair;
+                if (STYLE == 2) {
+                    wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
+                    wd = wd > (real)0 ? wd : (real)0;
+                    fpair = C.A * wc + C.B * (rhoi + L.rh[lk]) * wd;
+                } else {
+                    fpair = C.A * wc;
+                }
+                fpair -= C.gamma * wr * wr * dot * rinv;
+                fpair += C.sig * wr * rnd;
+                fpair *= rinv;
+                const real ffx = dx * fpair, ffy = dy * fpair, ffz = dz * fpair;
+                fx += (double)ffx;
+                fy += (double)ffy;
+                fz += (double)ffz;
+                if (log) {
+                    if (STYLE == 2) {
+                        const real u_c = (real)0.5 * C.A * C.rc * wc * wc;
+                        const real u_d = (real)0.25 * C.B * C.rd * rhoi * wd * wd;
+                        pe += (double)((real)0.5 * u_c + u_d);
+                    } else {
+                        const real ppe = (real)0.5 * C.A * C.rc * wc * wc;
+                        pe += (double)((real)0.5 * ppe);
+                    }
+                    w0 += (double)((real)0.5 * dx * ffx);
+                    w1 += (double)((real)0.5 * dx * ffy);
+                    w2 += (double)((real)0.5 * dx * ffz);
+                    w3 += (double)((real)0.5 * dy * ffy);
+                    w4 += (double)((real)0.5 * dy * ffz);
+                    w5 += (double)((real)0.5 * dz * ffz);
+                }
+            }
+            if (run == 9) break;
         }
 
         // phase 3: bonds, then store or integrate
@@ -655,7 +695,8 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         }
         double xg = X[slot], yg = X[N + slot], zg = X[2 * N + slot];
         double bpe = 0.0;
-        if (R.nbonds > 0) {
+        // bonds belong to Bond::compute_force on the hook path (bond_harmonic.cu:355-402)
+        if (R.nbonds > 0 && !force_only) {
             const int b1 = R.bond_start[idi];
             for (int bb = R.bond_start[idi - 1]; bb < b1; ++bb) {
                 const int ps = R.slot_of_id[R.bond_partner[bb] - 1];
@@ -701,10 +742,12 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         const int tt = L.sty[li] - 1;
         const double h = (0.5 * R.dt) * R.minv[tt];
         double vxg = V[slot], vyg = V[N + slot], vzg = V[2 * N + slot];
+        // explicit FMAs: the two half-kicks round exactly as two separate
+        // nve kernels do (fix/fix_nve.cu:54-56, :143-145)
         if (mode & kModeFinalKick) {
-            vxg = vxg + h * fx;
-            vyg = vyg + h * fy;
-            vzg = vzg + h * fz;
+            vxg = fma(h, fx, vxg);
+            vyg = fma(h, fy, vyg);
+            vzg = fma(h, fz, vzg);
         }
         if (log) {
             const double m = R.mass[tt];
@@ -719,16 +762,16 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
             acc[8] += m * vzg * vzg + w5;
         }
         if (mode & kModeDrift) {
-            vxg = vxg + h * fx;
-            vyg = vyg + h * fy;
-            vzg = vzg + h * fz;
+            vxg = fma(h, fx, vxg);
+            vyg = fma(h, fy, vyg);
+            vzg = fma(h, fz, vzg);
             int ix, iy, iz;
             unpack_img(R.img[slot], ix, iy, iz);
             const int ix0 = ix, iy0 = iy, iz0 = iz;
             bool b2 = false;
-            xg = wrap_axis(xg + R.dt * vxg, R.lo[0], R.lo[0] + R.L[0], R.L[0], ix, b2);
-            yg = wrap_axis(yg + R.dt * vyg, R.lo[1], R.lo[1] + R.L[1], R.L[1], iy, b2);
-            zg = wrap_axis(zg + R.dt * vzg, R.lo[2], R.lo[2] + R.L[2], R.L[2], iz, b2);
+            xg = wrap_axis(fma(R.dt, vxg, xg), R.lo[0], R.lo[0] + R.L[0], R.L[0], ix, b2);
+            yg = wrap_axis(fma(R.dt, vyg, yg), R.lo[1], R.lo[1] + R.L[1], R.L[1], iy, b2);
+            zg = wrap_axis(fma(R.dt, vzg, zg), R.lo[2], R.lo[2] + R.L[2], R.L[2], iz, b2);
             if (ix != ix0 || iy != iy0 || iz != iz0) R.img[slot] = pack_img(ix, iy, iz);
             if (b2 || !isfinite(vxg + vyg + vzg)) {
                 bad = true;
@@ -819,10 +862,10 @@ __global__ __launch_bounds__(kBlock) void k_nve(const RepDev *__restrict__ reps,
     bool bad = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        v[d] = V[d * N + s] + h * R.force[d * N + s];
+        v[d] = fma(h, R.force[d * N + s], V[d * N + s]);
         V[d * N + s] = v[d];
         if (initial) {
-            x[d] = wrap_axis(X[d * N + s] + R.dt * v[d], R.lo[d], R.lo[d] + R.L[d], R.L[d], im[d], bad);
+            x[d] = wrap_axis(fma(R.dt, v[d], X[d * N + s]), R.lo[d], R.lo[d] + R.L[d], R.L[d], im[d], bad);
             X[d * N + s] = x[d];
         }
     }
@@ -1051,8 +1094,7 @@ size_t BrickLaunch::lds_bytes(int precision, int style) const
 {
     (void)style;
     BrickParams p{rmax, smax, kmax, ncol, ntypes};
-    return precision == 1 ? brick_lds_carve<double>(p, nullptr, nullptr)
-                          : brick_lds_carve<float>(p, nullptr, nullptr);
+    return precision == 1 ? lds_offsets<double>(p).total : lds_offsets<float>(p).total;
 }
 
 void launch_bin_keys(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s)
