@@ -242,7 +242,7 @@ BrickLaunch brick_params_of(const Sys *s)
     BrickLaunch b{};
     b.rmax = (s->h.bdim[0] + 2) * (s->h.bdim[1] + 2) * (s->h.bdim[2] + 2);
     b.smax = s->h.smax;
-    b.kmax = s->h.kmax;
+    b.omax = s->h.omax;
     b.ncol = s->h.bdim[0] * s->h.bdim[1];
     b.ntypes = s->h.ntypes;
     return b;
@@ -252,7 +252,7 @@ void bl_max(BrickLaunch &a, const BrickLaunch &b)
 {
     a.rmax = std::max(a.rmax, b.rmax);
     a.smax = std::max(a.smax, b.smax);
-    a.kmax = std::max(a.kmax, b.kmax);
+    a.omax = std::max(a.omax, b.omax);
     a.ncol = std::max(a.ncol, b.ncol);
     a.ntypes = std::max(a.ntypes, b.ntypes);
 }
@@ -333,6 +333,7 @@ int rebin(gdpd_ctx *ctx, Tables &T)
 
 int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
 {
+    T.bl_dpd.perbead = T.bl_mdpd.perbead = ((mode & kModeForceOnly) && (mode & kModeLogForce)) ? 1 : 0;
     const bool timed = ctx->timing;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (T.n_mdpd) launch_density(ctx->d_reps, T.bricks_mdpd, T.n_mdpd, T.bl_mdpd, T.precision, ctx->stream);
@@ -355,7 +356,8 @@ int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
 int step_once(gdpd_ctx *ctx, Tables &T, int mode)
 {
     int rc;
-    if ((rc = force_launch(ctx, T, mode | kModeLogAuto))) return rc;
+    if ((rc = force_launch(ctx, T, kModeLogAuto))) return rc;
+    launch_integrate(ctx->d_reps, T.beads, T.n_beads, mode | kModeLogAuto, ctx->stream);
     launch_step_end(ctx->d_reps, T.rep_list, T.n_reps, 1, 1, 1, ctx->stream);
     if (mode & kModeDrift) {
         launch_scan(ctx->d_reps, T.scan, T.n_scan, T.rep_chunk_off, T.rep_list, T.n_reps, T.chunk_sums, ctx->stream);
@@ -378,9 +380,8 @@ int check_flags(gdpd_ctx *ctx, Sys *s)
     int rc = read_state(ctx, s, &st);
     if (rc) return rc;
     if (st.flags & kFlagStageOverflow)
-        return fail(ctx, GDPD_ERR_OVERFLOW, "a brick held more beads than its LDS capacity (smax=%d)", s->h.smax);
-    if (st.flags & kFlagListOverflow)
-        return fail(ctx, GDPD_ERR_OVERFLOW, "a bead had more than kmax=%d neighbours in range", s->h.kmax);
+        return fail(ctx, GDPD_ERR_OVERFLOW, "a brick held more beads than its LDS capacity (smax=%d, omax=%d)",
+                    s->h.smax, s->h.omax);
     if (st.flags & kFlagNonFinite)
         return fail(ctx, GDPD_ERR_NAN, "non-finite or runaway bead coordinates");
     return GDPD_OK;
@@ -392,15 +393,13 @@ int choose_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cell_start)
 {
     RepDev &h = s->h;
     // exact region counts of every brick at this state, then a margin for drift
-    int maxreg = 0;
-    double maxdens = 0.0;
-    const double cellvol = h.cs[0] * h.cs[1] * h.cs[2];
+    int maxreg = 0, maxown = 0;
     for (int b = 0; b < h.nbrick; ++b) {
         const int bz = b % h.nbk[2], by = (b / h.nbk[2]) % h.nbk[1], bx = b / (h.nbk[2] * h.nbk[1]);
         const int c0[3] = {bx * h.bdim[0], by * h.bdim[1], bz * h.bdim[2]};
         int ext[3];
         for (int d = 0; d < 3; ++d) ext[d] = std::min(h.bdim[d], h.nc[d] - c0[d]);
-        int cnt = 0, ncel = 0;
+        int cnt = 0, own = 0;
         for (int rx = -1; rx <= ext[0]; ++rx)
             for (int ry = -1; ry <= ext[1]; ++ry)
                 for (int rz = -1; rz <= ext[2]; ++rz) {
@@ -408,15 +407,16 @@ int choose_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cell_start)
                     const int gy = (c0[1] + ry + h.nc[1]) % h.nc[1];
                     const int gz = (c0[2] + rz + h.nc[2]) % h.nc[2];
                     const int c = (gx * h.nc[1] + gy) * h.nc[2] + gz;
-                    cnt += cell_start[c + 1] - cell_start[c];
-                    ++ncel;
+                    const int n = cell_start[c + 1] - cell_start[c];
+                    cnt += n;
+                    if (rx >= 0 && rx < ext[0] && ry >= 0 && ry < ext[1] && rz >= 0 && rz < ext[2]) own += n;
                 }
         maxreg = std::max(maxreg, cnt);
-        maxdens = std::max(maxdens, cnt / (ncel * cellvol));
+        maxown = std::max(maxown, own);
     }
-    h.smax = std::max(64, (int)std::ceil(maxreg * 1.35) + 64);
-    const double rcm = std::sqrt(h.rcmax2);
-    h.kmax = std::max(16, (int)std::ceil(maxdens * 4.0 / 3.0 * M_PI * rcm * rcm * rcm * 1.8) + 16);
+    // margins for density drift between re-checks
+    h.smax = std::min(65000, std::max(64, (int)std::ceil(maxreg * 1.12) + 32));
+    h.omax = std::min(65000, std::max(32, (int)std::ceil(maxown * 1.15) + 16));
     (void)ctx;
     return GDPD_OK;
 }
@@ -444,7 +444,7 @@ int prepare(gdpd_ctx *ctx, Sys *s)
     const double cellvol = h.cs[0] * h.cs[1] * h.cs[2];
     for (int d = 0; d < 3; ++d) h.bdim[d] = s->tune_b[d];
     if (s->tune_b[0] <= 0 || s->tune_b[1] <= 0 || s->tune_b[2] <= 0) {
-        const double target = 200.0 / std::max(rho * cellvol, 1e-9);
+        const double target = 240.0 / std::max(rho * cellvol, 1e-9);
         int e = std::max(1, (int)std::floor(std::cbrt(target)));
         h.bdim[0] = h.bdim[1] = e;
         h.bdim[2] = std::max(1, (int)std::floor(target / (e * e)));
@@ -484,7 +484,7 @@ int prepare(gdpd_ctx *ctx, Sys *s)
     h.rcmax2 = rc2max;
     h.rdmax2 = rd2max;
     h.smax = 64;
-    h.kmax = 16;
+    h.omax = 32;
     s->prepared = true;
     const int id = (int)(std::find(ctx->sys.begin(), ctx->sys.end(), s) - ctx->sys.begin());
     if ((rc = push_desc(ctx, id))) return rc;
@@ -740,6 +740,7 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     ALLOC(h.state, 1);
     h.thermo_cap = 1024;
     ALLOC(h.thermo_rows, (size_t)h.thermo_cap * 12);
+    ALLOC(h.thermo_bead, (size_t)((N + kBlock - 1) / kBlock) * kThermoParts);
     // empty bond CSR (bond_start[id] == 0 for all ids)
     ALLOC(h.bond_start, (size_t)N + 1);
 #undef ALLOC
@@ -1099,7 +1100,16 @@ int gdpd_pair_compute(gdpd_ctx *ctx, int sid, uint32_t step, int log)
     int rc = ensure_cells(ctx, s);
     if (rc) return rc;
     if ((rc = set_state_step(ctx, s, step, false))) return rc;
+    const long long N = s->h.N;
+    if (log) {
+        launch_fill_f64(s->h.pe, 0.0, N, ctx->stream);
+        launch_fill_f64(s->h.vir, 0.0, 6 * N, ctx->stream);
+    }
     if ((rc = force_launch(ctx, s->tab, kModeForceOnly | (log ? kModeLogForce : 0)))) return rc;
+    if (log) {
+        launch_fix_to_double(s->h.pe, N, s->h.precision, ctx->stream);
+        launch_fix_to_double(s->h.vir, 6 * N, s->h.precision, ctx->stream);
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return check_flags(ctx, s);
 }
@@ -1304,7 +1314,7 @@ int gdpd_get_tuning(gdpd_ctx *ctx, int sid, int *out8)
     out8[1] = s->h.bdim[1];
     out8[2] = s->h.bdim[2];
     out8[3] = s->h.smax;
-    out8[4] = s->h.kmax;
+    out8[4] = s->h.omax;
     out8[5] = s->resort_every > 0 ? s->resort_every : 16;
     out8[6] = s->graph_steps > 0 ? s->graph_steps : 8;
     out8[7] = s->h.nbrick;

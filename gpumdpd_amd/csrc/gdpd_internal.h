@@ -51,7 +51,7 @@ struct RepDev {
     int nc[3], ncell;
     int bdim[3], nbk[3], nbrick;
     int N, ntypes, style, precision;
-    int smax, kmax;            // LDS capacities: staged beads, list entries per bead
+    int smax, omax;            // LDS capacities: staged and owned beads per brick
     double rcmax2, rdmax2;     // largest rc^2 and rd^2 over type pairs
     int log_every;
     unsigned seed;
@@ -76,7 +76,8 @@ struct RepDev {
     int *bond_start, *bond_partner;  // CSR by id-1: partner ids (1-based)
     unsigned char *bond_type;        // per CSR entry, 0-based
     const void *coef;          // PairCoef<real>[ntypes * ntypes]
-    double *thermo_part;       // [nbrick][kThermoParts]
+    double *thermo_part;       // [nbrick][kThermoParts]: pair PE and virial of log steps
+    double *thermo_bead;       // [ceil(N/256)][kThermoParts]: KE, bond terms, m v v
     double *thermo_rows;       // [thermo_cap][12]
     int thermo_cap;
     RepState *state;

@@ -7,9 +7,11 @@ namespace gdpd {
 struct BrickLaunch {
     int rmax;      // max region cells ((bx+2)(by+2)(bz+2)) over the launch
     int smax;      // max staged beads
-    int kmax;      // list entries per owned bead
+    int omax;      // max owned beads of a brick
     int ncol;      // max inner columns (bx*by)
     int ntypes;    // max types
+    int qw = 256;  // pair-queue entries per wave
+    int perbead = 0; // per-bead PE/virial sums (hook path log steps)
     size_t lds_bytes(int precision, int style) const;
 };
 
@@ -27,6 +29,8 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s);
 void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
                   int precision, int style, int mode, hipStream_t s);
+// bonds + integration + re-binning, one thread per slot
+void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, hipStream_t s);
 // per replica (rep_list: nreps replica indices)
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
                      int thermo, hipStream_t s);
@@ -47,6 +51,7 @@ void launch_nl_check(const RepDev *rep, int N, double skin, const double *last_p
                      hipStream_t s);
 void launch_saru_eval(int n, const uint32_t *t1, const uint32_t *t2, uint32_t seed, const int32_t *steps,
                       uint32_t *bits, float *r32, double *r64, hipStream_t s);
+void launch_fix_to_double(double *p, long long n, int precision, hipStream_t s);
 // fill helpers (graph-safe replacements for hipMemsetAsync)
 void launch_fill_i32(int *p, int v, long long n, hipStream_t s);
 void launch_fill_f64(double *p, double v, long long n, hipStream_t s);

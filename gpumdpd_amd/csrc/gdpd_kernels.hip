@@ -127,6 +127,41 @@ __device__ __forceinline__ void block_sum(double (&acc)[NV], double *red, double
     __syncthreads();
 }
 
+// Thermo partial sums as 64-bit fixed point (2^-32 resolution, rounded to
+// nearest, |sum| < 2^31): integer sums are exact, so a thermo record does not
+// depend on how beads are split over workgroups (slot layout, brick shape or
+// batch).
+constexpr double kThermoFix = 4294967296.0;
+
+__device__ __forceinline__ long long wave_sum_i64(long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int lo = __shfl_xor((int)(unsigned)(v & 0xFFFFFFFFll), o, 64);
+        const int hi = __shfl_xor((int)(v >> 32), o, 64);
+        v += ((long long)hi << 32) | (unsigned)lo;
+    }
+    return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum_fix(const double (&acc)[NV], long long *red, long long *out)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const long long s = wave_sum_i64(__double2ll_rn(acc[k] * kThermoFix));
+        if (lane == 0) red[w * NV + k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        long long s = 0;
+        for (int ww = 0; ww < kBlock / 64; ++ww) s += red[ww * NV + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // binning: keys, scan, scatter, in-cell sort, physical re-sort
 // ---------------------------------------------------------------------------
@@ -287,30 +322,49 @@ __global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restri
 }
 
 // ---------------------------------------------------------------------------
-// brick staging shared by the density and force kernels
+// brick kernels: LDS staging, per-wave pair queue, fixed-point accumulation
 // ---------------------------------------------------------------------------
+//
+// One 256-thread workgroup per brick of cells.  The brick and a one-cell halo
+// are staged into LDS (positions relative to the brick corner, periodic images
+// applied while staging).  Each pair of beads is evaluated once inside a brick:
+// an owned bead scans only the staged beads after it in LDS order plus every
+// halo bead, so a pair of owned beads is found by its lower bead alone and a
+// pair with a halo bead once per brick (the neighbouring brick evaluates its
+// own side).  Candidate tests run as wave-uniform loops that append in-range
+// pairs to a per-wave queue (ballot + mbcnt); full queues are evaluated with
+// every lane busy.  Per-bead sums are 64-bit fixed point in LDS: integer
+// additions are exact and commute, so the result does not depend on the order
+// in which waves and lanes deliver their pairs, nor on the batch or the slot
+// layout.  Per-pair arithmetic follows force/pair_dpd.cu:296-399 and
+// force/pair_mdpd.cu:219-300, 403-517.
 
 struct BrickParams {
-    int rmax, smax, kmax, ncol, ntypes;
+    int rmax, smax, omax, ncol, ntypes, qw, perbead, style;
 };
 
+template <typename real> struct Vec4T;
+template <> struct Vec4T<float> { using type = float4; };
+template <> struct Vec4T<double> { using type = double4; };
+
 template <typename real> struct BrickLds {
+    using real4 = typename Vec4T<real>::type;
     PairCoef<real> *coef;
     double *red;
-    real *px, *py, *pz, *vx, *vy, *vz, *rh;
-    int *c_start, *c_src, *sid, *colpre, *wbuf;
-    unsigned short *scell, *list;
-    unsigned char *sty;
+    real4 *pos, *vel;          // xyz + type, vxyz + id bits
+    real *rh;                  // staged density (mDPD force pass)
+    unsigned short *own;       // owned index, or 0xFFFF for halo beads
+    int *c_start, *c_src, *gstart, *colpre, *wbuf, *oslot, *oinfo;
+    long long *facc;           // fixed-point sums per owned bead
+    unsigned *queue;           // per wave: li | lj << 16
 };
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
-// LDS layout of the brick kernels as byte offsets.  The pointers are rebuilt
-// inline from the __shared__ base in every kernel so they stay in the LDS
-// address space (ds_read/ds_write), never in generic pointers or scratch.
+// LDS layout as byte offsets; pointers are rebuilt inline from the __shared__
+// base so they stay in the LDS address space.
 struct LdsOffsets {
-    unsigned coef, red, px, py, pz, vx, vy, vz, rh, c_start, c_src, sid, colpre, wbuf, scell, list, sty,
-        total;
+    unsigned coef, red, pos, vel, rh, own, c_start, c_src, gstart, colpre, wbuf, oslot, oinfo, facc, queue, total;
 };
 
 template <typename real>
@@ -323,21 +377,19 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     off = align16(off + (size_t)(bytes));
     GDPD_TAKE(coef, sizeof(PairCoef<real>) * p.ntypes * p.ntypes)
     GDPD_TAKE(red, sizeof(double) * 64)
-    GDPD_TAKE(px, sizeof(real) * p.smax)
-    GDPD_TAKE(py, sizeof(real) * p.smax)
-    GDPD_TAKE(pz, sizeof(real) * p.smax)
-    GDPD_TAKE(vx, sizeof(real) * p.smax)
-    GDPD_TAKE(vy, sizeof(real) * p.smax)
-    GDPD_TAKE(vz, sizeof(real) * p.smax)
-    GDPD_TAKE(rh, sizeof(real) * p.smax)
+    GDPD_TAKE(pos, sizeof(real) * 4 * p.smax)
+    GDPD_TAKE(vel, sizeof(real) * 4 * p.smax)
+    GDPD_TAKE(rh, sizeof(real) * (p.style == 2 ? p.smax : 1))
+    GDPD_TAKE(own, sizeof(unsigned short) * p.smax)
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
     GDPD_TAKE(c_src, sizeof(int) * p.rmax)
-    GDPD_TAKE(sid, sizeof(int) * p.smax)
+    GDPD_TAKE(gstart, sizeof(int) * p.rmax)
     GDPD_TAKE(colpre, sizeof(int) * (p.ncol + 1))
     GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
-    GDPD_TAKE(scell, sizeof(unsigned short) * p.smax)
-    GDPD_TAKE(list, sizeof(unsigned short) * (size_t)p.kmax * kBlock)
-    GDPD_TAKE(sty, sizeof(unsigned char) * p.smax)
+    GDPD_TAKE(oslot, sizeof(int) * p.omax)
+    GDPD_TAKE(oinfo, sizeof(int) * p.omax)
+    GDPD_TAKE(facc, sizeof(long long) * 3 * p.omax)
+    GDPD_TAKE(queue, sizeof(unsigned) * (kBlock / 64) * p.qw)
 #undef GDPD_TAKE
     o.total = (unsigned)off;
     return o;
@@ -346,24 +398,23 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
 template <typename real>
 __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets &o)
 {
+    using real4 = typename Vec4T<real>::type;
     BrickLds<real> L;
     L.coef = (PairCoef<real> *)(smem + o.coef);
     L.red = (double *)(smem + o.red);
-    L.px = (real *)(smem + o.px);
-    L.py = (real *)(smem + o.py);
-    L.pz = (real *)(smem + o.pz);
-    L.vx = (real *)(smem + o.vx);
-    L.vy = (real *)(smem + o.vy);
-    L.vz = (real *)(smem + o.vz);
+    L.pos = (real4 *)(smem + o.pos);
+    L.vel = (real4 *)(smem + o.vel);
     L.rh = (real *)(smem + o.rh);
+    L.own = (unsigned short *)(smem + o.own);
     L.c_start = (int *)(smem + o.c_start);
     L.c_src = (int *)(smem + o.c_src);
-    L.sid = (int *)(smem + o.sid);
+    L.gstart = (int *)(smem + o.gstart);
     L.colpre = (int *)(smem + o.colpre);
     L.wbuf = (int *)(smem + o.wbuf);
-    L.scell = (unsigned short *)(smem + o.scell);
-    L.list = (unsigned short *)(smem + o.list);
-    L.sty = (unsigned char *)(smem + o.sty);
+    L.oslot = (int *)(smem + o.oslot);
+    L.oinfo = (int *)(smem + o.oinfo);
+    L.facc = (long long *)(smem + o.facc);
+    L.queue = (unsigned *)(smem + o.queue);
     return L;
 }
 
@@ -404,12 +455,19 @@ __device__ __forceinline__ int wrap_cell(int c, int n, double L, double &shift)
     return c;
 }
 
-// Stage the brick + halo into LDS.  Returns the number of staged beads, or -1
-// (uniformly) when it does not fit.  need: 1 = velocities + ids, 2 = density.
+template <typename real> __device__ __forceinline__ real type_to_real(int t) { return (real)t; }
+__device__ __forceinline__ float id_to_real(int id, float) { return __int_as_float(id); }
+__device__ __forceinline__ double id_to_real(int id, double) { return __longlong_as_double((long long)id); }
+__device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_longlong(v); }
+
+// Stage the brick + halo.  Returns (uniformly) the number of owned beads, or
+// -1 when the staged or owned beads exceed the LDS capacity.
 template <typename real>
 __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
-                           const BrickParams &bp, int cur, bool need_vel, bool need_rho)
+                                           const BrickParams &bp, int cur, bool need_vel, bool need_rho)
 {
+    using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x;
     for (int rc = t; rc < g.nreg; rc += kBlock) {
         const int rz = rc % g.RZ, ry = (rc / g.RZ) % g.RY, rx = rc / (g.RZ * g.RY);
@@ -418,8 +476,10 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         const int gy = wrap_cell(g.c0[1] - 1 + ry, R.nc[1], R.L[1], sh);
         const int gz = wrap_cell(g.c0[2] - 1 + rz, R.nc[2], R.L[2], sh);
         const int gc = (gx * R.nc[1] + gy) * R.nc[2] + gz;
+        const int a = R.cell_start[gc], b = R.cell_start[gc + 1];
         L.c_src[rc] = gc;
-        L.c_start[rc + 1] = R.cell_start[gc + 1] - R.cell_start[gc];
+        L.gstart[rc] = a;
+        L.c_start[rc + 1] = b - a;
     }
     __syncthreads();
     {
@@ -438,11 +498,21 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             if (rc < g.nreg) L.c_start[rc] = run;
             run += v[k];
         }
-        if (t == 0) L.c_start[g.nreg] = total;
+        if (t == 0) {
+            L.c_start[g.nreg] = total;
+            // owned beads: brick columns (rx, ry) = 1..ext, cells rz = 1..ext[2]
+            L.colpre[0] = 0;
+            const int ncol = g.ext[0] * g.ext[1];
+            for (int q = 0; q < ncol; ++q) {
+                const int c = ((1 + q / g.ext[1]) * g.RY + (1 + q % g.ext[1])) * g.RZ;
+                L.colpre[q + 1] = L.colpre[q] + (L.c_start[c + g.ext[2] + 1] - L.c_start[c + 1]);
+            }
+        }
     }
     __syncthreads();
     const int S = L.c_start[g.nreg];
-    if (S > bp.smax) return -1;
+    const int O = L.colpre[g.ext[0] * g.ext[1]];
+    if (S > bp.smax || O > bp.omax) return -1;
 
     const int N = R.N;
     const double *X = R.x[cur], *V = R.v[cur];
@@ -455,60 +525,236 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         }
         const int rc = lo;
         const int rz = rc % g.RZ, ry = (rc / g.RZ) % g.RY, rx = rc / (g.RZ * g.RY);
-        const int gc = L.c_src[rc];
-        int slot = R.perm[R.cell_start[gc] + (k - L.c_start[rc])];
+        int slot = R.perm[L.gstart[rc] + (k - L.c_start[rc])];
         if ((unsigned)slot >= (unsigned)N) slot = 0;
         double shx, shy, shz;
         wrap_cell(g.c0[0] - 1 + rx, R.nc[0], R.L[0], shx);
         wrap_cell(g.c0[1] - 1 + ry, R.nc[1], R.L[1], shy);
         wrap_cell(g.c0[2] - 1 + rz, R.nc[2], R.L[2], shz);
-        L.px[k] = (real)((X[slot] + shx) - g.org[0]);
-        L.py[k] = (real)((X[N + slot] + shy) - g.org[1]);
-        L.pz[k] = (real)((X[2 * N + slot] + shz) - g.org[2]);
-        L.sty[k] = R.type[slot];
-        L.scell[k] = (unsigned short)rc;
+        real4 p;
+        p.x = (real)((X[slot] + shx) - g.org[0]);
+        p.y = (real)((X[N + slot] + shy) - g.org[1]);
+        p.z = (real)((X[2 * N + slot] + shz) - g.org[2]);
+        const bool inner = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
+        // type in w, negative for halo beads
+        p.w = type_to_real<real>(inner ? (int)R.type[slot] : -(int)R.type[slot]);
+        L.pos[k] = p;
         if (need_vel) {
-            L.vx[k] = (real)V[slot];
-            L.vy[k] = (real)V[N + slot];
-            L.vz[k] = (real)V[2 * N + slot];
-            L.sid[k] = R.id[slot];
+            real4 q;
+            q.x = (real)V[slot];
+            q.y = (real)V[N + slot];
+            q.z = (real)V[2 * N + slot];
+            q.w = id_to_real(R.id[slot], (real)0);
+            L.vel[k] = q;
         }
         if (need_rho) L.rh[k] = (real)R.rho[slot];
-    }
-    const int ncol = g.ext[0] * g.ext[1];
-    for (int k = t; k < R.ntypes * R.ntypes; k += kBlock)
-        L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
-    if (t == 0) {
-        L.colpre[0] = 0;
-        for (int q = 0; q < ncol; ++q) {
-            const int rx = 1 + q / g.ext[1], ry = 1 + q % g.ext[1];
-            const int c = (rx * g.RY + ry) * g.RZ;
-            L.colpre[q + 1] = L.colpre[q] + (L.c_start[c + g.ext[2] + 1] - L.c_start[c + 1]);
+        unsigned short ow = 0xFFFF;
+        if (inner) {
+            const int q = (rx - 1) * g.ext[1] + (ry - 1);
+            const int o = L.colpre[q] + (k - L.c_start[(rx * g.RY + ry) * g.RZ + 1]);
+            ow = (unsigned short)o;
+            L.oslot[o] = slot;
+            L.oinfo[o] = k | (rc << 16);
         }
+        L.own[k] = ow;
     }
+    for (int k = t; k < R.ntypes * R.ntypes; k += kBlock) L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
+    for (int k = t; k < 3 * O; k += kBlock) L.facc[k] = 0;
     __syncthreads();
-    return S;
-}
-
-// owned bead o -> staged index (brick cells are the region minus its halo)
-__device__ __forceinline__ int owned_to_staged(const BrickGeom &g, const int *colpre,
-                                               const int *c_start, int o, int &rx, int &ry)
-{
-    int q = 0;
-    while (colpre[q + 1] <= o) ++q;
-    rx = 1 + q / g.ext[1];
-    ry = 1 + q % g.ext[1];
-    return c_start[(rx * g.RY + ry) * g.RZ + 1] + (o - colpre[q]);
+    return O;
 }
 
 template <typename real> __device__ __forceinline__ real ratio(real a, real b, real binv);
 template <> __device__ __forceinline__ float ratio<float>(float a, float b, float binv) { return a * binv; }
 template <> __device__ __forceinline__ double ratio<double>(double a, double b, double binv) { return a / b; }
 
+// fixed-point scale of the per-bead sums: 2^32 for FP32 pair arithmetic,
+// 2^40 (|sum| < 8.3e6) for FP64
+template <typename real> struct FixScale;
+template <> struct FixScale<float> { static constexpr double s = 4294967296.0, inv = 1.0 / 4294967296.0; };
+template <> struct FixScale<double> { static constexpr double s = 1099511627776.0, inv = 1.0 / 1099511627776.0; };
+
+template <typename real> __device__ __forceinline__ long long to_fix(double v)
+{
+    return __double2ll_rn(v * FixScale<real>::s);
+}
+
+__device__ __forceinline__ void lds_add(long long *p, long long v)
+{
+    atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Thermo partial sums per thread: 0 ke, 1 pair pe, 2 bond pe, 3..8 pressure tensor.
+template <typename real, int STYLE, bool DENSITY>
+__device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real> &L, const unsigned *q, int qn,
+                                           int nt, uint32_t seed_step, bool log, bool perbead,
+                                           double (&acc)[kThermoParts])
+{
+    using real4 = typename Vec4T<real>::type;
+    const int lane = threadIdx.x & 63;
+    for (int e = lane; e < qn; e += 64) {
+        const unsigned ent = q[e];
+        const int li = ent & 0xFFFF, lj = ent >> 16;
+        const real4 pi = L.pos[li], pj = L.pos[lj];
+        const int oi = L.own[li], oj = L.own[lj];
+        const bool jown = oj != 0xFFFF;
+        const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const real r2 = dx * dx + dy * dy + dz * dz;
+        const PairCoef<real> &C = L.coef[((int)fabs(pi.w) - 1) * nt + ((int)fabs(pj.w) - 1)];
+        const real r = sqrt(r2);
+        if (DENSITY) {
+            // force/pair_mdpd.cu:266-289
+            real wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
+            wd = wd > (real)0 ? wd : (real)0;
+            const long long w = to_fix<real>((double)(C.rhofac * wd * wd));
+            lds_add(&L.facc[oi], w);
+            if (jown) lds_add(&L.facc[oj], w);
+            continue;
+        }
+        if (r2 > C.rc2) continue;
+        const real4 vi = L.vel[li], vj = L.vel[lj];
+        const real rinv = (real)1 / r;
+        const real dot = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
+        const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
+        const real wr = wc;
+        const int idi = real_to_id(vi.w), idj = real_to_id(vj.w);
+        const uint32_t t1 = (uint32_t)min(idi, idj), t2 = (uint32_t)max(idi, idj);
+        const real rnd = saru_uniform<real>(saru_bits(t1, t2, seed_step));
+        real wd = (real)0, fpair;
+        real rhoi = (real)0, rhoj = (real)0;
+        if (STYLE == 2) {
+            rhoi = L.rh[li];
+            rhoj = L.rh[lj];
+            wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
+            wd = wd > (real)0 ? wd : (real)0;
+            fpair = C.A * wc + C.B * (rhoi + rhoj) * wd;
+        } else {
+            fpair = C.A * wc;
+        }
+        fpair -= C.gamma * wr * wr * dot * rinv;
+        fpair += C.sig * wr * rnd;
+        fpair *= rinv;
+        const real ffx = dx * fpair, ffy = dy * fpair, ffz = dz * fpair;
+        const long long qx = to_fix<real>((double)ffx), qy = to_fix<real>((double)ffy), qz = to_fix<real>((double)ffz);
+        lds_add(&L.facc[3 * oi + 0], qx);
+        lds_add(&L.facc[3 * oi + 1], qy);
+        lds_add(&L.facc[3 * oi + 2], qz);
+        if (jown) {
+            lds_add(&L.facc[3 * oj + 0], -qx);
+            lds_add(&L.facc[3 * oj + 1], -qy);
+            lds_add(&L.facc[3 * oj + 2], -qz);
+        }
+        if (log) {
+            // per-bead shares as in force/pair_dpd.cu:366-398 and pair_mdpd.cu:479-516
+            real pe_i, pe_j;
+            if (STYLE == 2) {
+                const real u_c = (real)0.5 * C.A * C.rc * wc * wc;
+                pe_i = (real)0.5 * u_c + (real)0.25 * C.B * C.rd * rhoi * wd * wd;
+                pe_j = (real)0.5 * u_c + (real)0.25 * C.B * C.rd * rhoj * wd * wd;
+            } else {
+                const real ppe = (real)0.5 * C.A * C.rc * wc * wc;
+                pe_i = pe_j = (real)0.5 * ppe;
+            }
+            const real w6[6] = {(real)0.5 * dx * ffx, (real)0.5 * dx * ffy, (real)0.5 * dx * ffz,
+                                (real)0.5 * dy * ffy, (real)0.5 * dy * ffz, (real)0.5 * dz * ffz};
+            const double sides = jown ? 2.0 : 1.0;
+            acc[1] += (double)pe_i + (jown ? (double)pe_j : 0.0);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[3 + c] += sides * (double)w6[c];
+            if (perbead) {
+                // hook-path log step: per-bead fixed-point sums in the pe/virial arrays
+                const int N = R.N, si = L.oslot[oi], sj = jown ? L.oslot[oj] : 0;
+                atomicAdd((unsigned long long *)&R.pe[si], (unsigned long long)to_fix<real>((double)pe_i));
+                if (jown) atomicAdd((unsigned long long *)&R.pe[sj], (unsigned long long)to_fix<real>((double)pe_j));
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    const unsigned long long wq = (unsigned long long)to_fix<real>((double)w6[c]);
+                    atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + si], wq);
+                    if (jown) atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + sj], wq);
+                }
+            }
+        }
+    }
+}
+
+// Generation + evaluation.  Wave w takes owned beads in chunks of 64 (chunks
+// w, w+4, ...).  Each lane scans its bead's 27 neighbour cells as 9 z-runs of 3
+// cells (contiguous in LDS); a candidate j is kept when it is in range and
+// either lies after the bead in LDS order or is a halo bead, so a pair of owned
+// beads is found once and a pair with a halo bead once per brick.
+template <typename real, int STYLE, bool DENSITY>
+__device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
+                                            const BrickParams &bp, int O, int nt, real cut2, uint32_t seed_step,
+                                            bool log, double (&acc)[kThermoParts])
+{
+    using real4 = typename Vec4T<real>::type;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int qw = bp.qw;
+    unsigned *q = L.queue + w * qw;
+    int qn = 0;
+    for (int chunk = w; chunk * 64 < O; chunk += kBlock / 64) {
+        const int o = chunk * 64 + lane;
+        const bool valid = o < O;
+        int li = 0, rx = 1, ry = 1, rz = 1;
+        real4 pi;
+        pi.x = pi.y = pi.z = pi.w = (real)0;
+        if (valid) {
+            const int info = L.oinfo[o];
+            li = info & 0xFFFF;
+            const int rc = info >> 16;
+            rz = rc % g.RZ;
+            ry = (rc / g.RZ) % g.RY;
+            rx = rc / (g.RZ * g.RY);
+            pi = L.pos[li];
+        }
+        for (int r = 0; r < 9; ++r) {
+            int js = 0, len = 0;
+            if (valid) {
+                const int cb = ((rx + r / 3 - 1) * g.RY + (ry + r % 3 - 1)) * g.RZ + rz - 1;
+                js = L.c_start[cb];
+                len = L.c_start[cb + 3] - js;
+            }
+            const int maxlen = wave_max(len);
+            for (int k = 0; k < maxlen; ++k) {
+                if (qn > qw - 64) {
+                    eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
+                    qn = 0;
+                }
+                bool pass = false;
+                const int lj = js + k;
+                if (k < len) {
+                    const real4 pj = L.pos[lj];
+                    const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const real r2 = dx * dx + dy * dy + dz * dz;
+                    const bool inr = DENSITY ? (r2 < cut2) : (r2 <= cut2 && r2 >= (real)1.0e-20);
+                    pass = inr && (lj > li || pj.w < (real)0);
+                }
+                const unsigned long long m = __ballot(pass);
+                if (pass) q[qn + lanes_below(m)] = (unsigned)li | ((unsigned)lj << 16);
+                qn += __popcll(m);
+            }
+        }
+    }
+    eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // mDPD pass 1: local density (force/pair_mdpd.cu:219-300).  Every pair j != i
-// inside rd contributes (15/(2 pi rd^3)) (1 - r/rd)^2; there is no rc test and
-// no self term.  Summed per bead in FP64 in a fixed order.
+// inside rd contributes (15/(2 pi rd^3)) (1 - r/rd)^2 to both beads; there is
+// no rc test and no self term.
 // ---------------------------------------------------------------------------
 
 template <typename real>
@@ -522,48 +768,22 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     const RepDev &R = reps[it.rep];
     const int cur = R.state->cur;
     const BrickGeom g = brick_geom(R, it.local);
-    const int S = stage_brick<real>(R, g, L, bp, cur, false, false);
-    if (S < 0) {
+    const int O = stage_brick<real>(R, g, L, bp, cur, false, false);
+    if (O < 0) {
         if (threadIdx.x == 0) atomicOr(&R.state->flags, kFlagStageOverflow);
         return;
     }
-    const int O = L.colpre[g.ext[0] * g.ext[1]];
-    const int nt = R.ntypes;
-    const real rd2 = (real)R.rdmax2;
-    for (int o = threadIdx.x; o < O; o += kBlock) {
-        int rx, ry;
-        const int li = owned_to_staged(g, L.colpre, L.c_start, o, rx, ry);
-        const int rc = L.scell[li], rz = rc % g.RZ;
-        const real xi = L.px[li], yi = L.py[li], zi = L.pz[li];
-        const int ti = L.sty[li] - 1;
-        double rho = 0.0;
-        for (int ox = -1; ox <= 1; ++ox)
-            for (int oy = -1; oy <= 1; ++oy) {
-                const int base = ((rx + ox) * g.RY + (ry + oy)) * g.RZ + rz - 1;
-                const int j1 = L.c_start[base + 3];
-                for (int lj = L.c_start[base]; lj < j1; ++lj) {
-                    const real dx = xi - L.px[lj], dy = yi - L.py[lj], dz = zi - L.pz[lj];
-                    const real r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 < rd2 && lj != li) {
-                        const PairCoef<real> &C = L.coef[ti * nt + (L.sty[lj] - 1)];
-                        const real r = sqrt(r2);
-                        real wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
-                        wd = wd > (real)0 ? wd : (real)0;
-                        rho += (double)(C.rhofac * wd * wd);
-                    }
-                }
-            }
-        const int gc = L.c_src[rc];
-        const int slot = R.perm[R.cell_start[gc] + (li - L.c_start[rc])];
-        R.rho[slot] = rho;
-    }
+    double acc[kThermoParts];
+#pragma unroll
+    for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
+    brick_pairs<real, 2, true>(R, g, L, bp, O, R.ntypes, (real)R.rdmax2, 0u, false, acc);
+    for (int o = threadIdx.x; o < O; o += kBlock)
+        R.rho[L.oslot[o]] = (double)L.facc[o] * FixScale<real>::inv;
 }
 
 // ---------------------------------------------------------------------------
-// pair force (+ bonds) and, unless kModeForceOnly, the fused integration.
-// Pair arithmetic follows force/pair_dpd.cu:296-399 and pair_mdpd.cu:403-517;
-// bonds follow force/bond_harmonic.cu:196-268; integration fix/fix_nve.cu:48-86
-// and :137-149; thermo sums compute/thermo.cu:243-245, :328, :424-433.
+// pair force of every owned bead of a brick, written to the force array, with
+// the pair PE/virial of log steps (per brick, or per bead on the hook path)
 // ---------------------------------------------------------------------------
 
 template <typename real, int STYLE>
@@ -579,124 +799,64 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     const long long step = R.state->step;
     const bool log = (mode & kModeLogForce) ||
                      ((mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0);
-    const bool force_only = (mode & kModeForceOnly) != 0;
     const BrickGeom g = brick_geom(R, it.local);
-    const int S = stage_brick<real>(R, g, L, bp, cur, true, STYLE == 2);
-    if (S < 0) {
+    const int O = stage_brick<real>(R, g, L, bp, cur, true, STYLE == 2);
+    if (O < 0) {
         if (threadIdx.x == 0) atomicOr(&R.state->flags, kFlagStageOverflow);
         return;
     }
     const int t = threadIdx.x;
-    const int N = R.N, nt = R.ntypes, kmax = max(bp.kmax, 1);
-    const int O = L.colpre[g.ext[0] * g.ext[1]];
-    const real rcut2 = (real)R.rcmax2;
+    const int N = R.N;
     const uint32_t seed_step = R.seed + (uint32_t)step;
-    const double *X = R.x[cur], *V = R.v[cur];
+    double acc[kThermoParts];
+#pragma unroll
+    for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
+    brick_pairs<real, STYLE, false>(R, g, L, bp, O, R.ntypes, (real)R.rcmax2, seed_step, log, acc);
+    for (int o = t; o < O; o += kBlock) {
+        const int slot = L.oslot[o];
+        R.force[slot] = (double)L.facc[3 * o + 0] * FixScale<real>::inv;
+        R.force[N + slot] = (double)L.facc[3 * o + 1] * FixScale<real>::inv;
+        R.force[2 * N + slot] = (double)L.facc[3 * o + 2] * FixScale<real>::inv;
+        if (log && bp.perbead) R.bpe[slot] = 0.0;
+    }
+    if (log && !bp.perbead) {
+        long long *red = (long long *)L.red, *out = red + 48;
+        block_sum_fix<kThermoParts>(acc, red, out);
+        if (t < kThermoParts) ((long long *)R.thermo_part)[(size_t)it.local * kThermoParts + t] = out[t];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused bonds + integration, one thread per slot: bond forces
+// (force/bond_harmonic.cu:196-268) are added to the pair force, then the final
+// half-kick of this step (fix/fix_nve.cu:137-149), the thermo sums of a log step
+// (compute/thermo.cu:243-245, :424-433), the half-kick and drift of the next step
+// (fix/fix_nve.cu:48-86) and the new cell key.  The new state goes to the other
+// buffer.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__ reps,
+                                                      const WorkItem *__restrict__ items, int nitems, int mode)
+{
+    __shared__ long long red[64];
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int cur = R.state->cur;
+    const long long step = R.state->step;
+    const bool log = (mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0;
+    const int N = R.N;
+    const int s = it.local * kBlock + threadIdx.x;
     double acc[kThermoParts];
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
     bool bad = false;
-
-    for (int o = t; o < O; o += kBlock) {
-        int rx, ry;
-        const int li = owned_to_staged(g, L.colpre, L.c_start, o, rx, ry);
-        const int rc = L.scell[li], rz = rc % g.RZ;
-        const real xi = L.px[li], yi = L.py[li], zi = L.pz[li];
-
-        const int idi = L.sid[li];
-        const int ti = L.sty[li] - 1;
-        const real vxi = L.vx[li], vyi = L.vy[li], vzi = L.vz[li];
-        const real rhoi = (STYLE == 2) ? L.rh[li] : (real)0;
-        double fx = 0.0, fy = 0.0, fz = 0.0, pe = 0.0;
-        double w0 = 0.0, w1 = 0.0, w2 = 0.0, w3 = 0.0, w4 = 0.0, w5 = 0.0;
-        // The 27 neighbour cells are 9 z-runs of 3 cells, contiguous in LDS.
-        // Phase 1 appends in-range neighbours to this thread's list; when the
-        // list is full, phase 2 evaluates it and phase 1 resumes at its cursor,
-        // so the summation order is the candidate order whatever kmax is.
-        int run = 0, lj = 0, j1 = 0;
-        {
-            const int base = ((rx - 1) * g.RY + (ry - 1)) * g.RZ + rz - 1;
-            lj = L.c_start[base];
-            j1 = L.c_start[base + 3];
-        }
-        while (true) {
-            int cnt = 0;
-            while (run < 9) {
-                if (lj >= j1) {
-                    if (++run == 9) break;
-                    const int base = ((rx - 1 + run / 3) * g.RY + (ry - 1 + run % 3)) * g.RZ + rz - 1;
-                    lj = L.c_start[base];
-                    j1 = L.c_start[base + 3];
-                    continue;
-                }
-                if (cnt == kmax) break;
-                const real dx = xi - L.px[lj], dy = yi - L.py[lj], dz = zi - L.pz[lj];
-                const real r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= rcut2 && r2 >= (real)1.0e-20 && lj != li) L.list[(cnt++) * kBlock + t] = (unsigned short)lj;
-                ++lj;
-            }
-
-            for (int k = 0; k < cnt; ++k) {
-                const int lk = L.list[k * kBlock + t];
-                const real dx = xi - L.px[lk], dy = yi - L.py[lk], dz = zi - L.pz[lk];
-                const real r2 = dx * dx + dy * dy + dz * dz;
-                const PairCoef<real> &C = L.coef[ti * nt + (L.sty[lk] - 1)];
-                if (r2 > C.rc2) continue;
-                const real r = sqrt(r2);
-                const real rinv = (real)1 / r;
-                const real dot = dx * (vxi - L.vx[lk]) + dy * (vyi - L.vy[lk]) + dz * (vzi - L.vz[lk]);
-                const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
-                const real wr = wc;
-                const int idj = L.sid[lk];
-                const uint32_t t1 = (uint32_t)min(idi, idj), t2 = (uint32_t)max(idi, idj);
-                const real rnd = saru_uniform<real>(saru_bits(t1, t2, seed_step));
-                real wd = (real)0;
-                real fpair;
-                if (STYLE == 2) {
-                    wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
-                    wd = wd > (real)0 ? wd : (real)0;
-                    fpair = C.A * wc + C.B * (rhoi + L.rh[lk]) * wd;
-                } else {
-                    fpair = C.A * wc;
-                }
-                fpair -= C.gamma * wr * wr * dot * rinv;
-                fpair += C.sig * wr * rnd;
-                fpair *= rinv;
-                const real ffx = dx * fpair, ffy = dy * fpair, ffz = dz * fpair;
-                fx += (double)ffx;
-                fy += (double)ffy;
-                fz += (double)ffz;
-                if (log) {
-                    if (STYLE == 2) {
-                        const real u_c = (real)0.5 * C.A * C.rc * wc * wc;
-                        const real u_d = (real)0.25 * C.B * C.rd * rhoi * wd * wd;
-                        pe += (double)((real)0.5 * u_c + u_d);
-                    } else {
-                        const real ppe = (real)0.5 * C.A * C.rc * wc * wc;
-                        pe += (double)((real)0.5 * ppe);
-                    }
-                    w0 += (double)((real)0.5 * dx * ffx);
-                    w1 += (double)((real)0.5 * dx * ffy);
-                    w2 += (double)((real)0.5 * dx * ffz);
-                    w3 += (double)((real)0.5 * dy * ffy);
-                    w4 += (double)((real)0.5 * dy * ffz);
-                    w5 += (double)((real)0.5 * dz * ffz);
-                }
-            }
-            if (run == 9) break;
-        }
-
-        // phase 3: bonds, then store or integrate
-        const int gc = L.c_src[rc];
-        int slot = R.perm[R.cell_start[gc] + (li - L.c_start[rc])];
-        if ((unsigned)slot >= (unsigned)N) {
-            bad = true;
-            continue;
-        }
-        double xg = X[slot], yg = X[N + slot], zg = X[2 * N + slot];
-        double bpe = 0.0;
-        // bonds belong to Bond::compute_force on the hook path (bond_harmonic.cu:355-402)
-        if (R.nbonds > 0 && !force_only) {
+    if (s < N) {
+        const double *X = R.x[cur], *V = R.v[cur];
+        double fx = R.force[s], fy = R.force[N + s], fz = R.force[2 * N + s];
+        double xg = X[s], yg = X[N + s], zg = X[2 * N + s];
+        double bpe = 0.0, bw[6] = {0, 0, 0, 0, 0, 0};
+        if (R.nbonds > 0) {
+            const int idi = R.id[s];
             const int b1 = R.bond_start[idi];
             for (int bb = R.bond_start[idi - 1]; bb < b1; ++bb) {
                 const int ps = R.slot_of_id[R.bond_partner[bb] - 1];
@@ -713,35 +873,23 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
                 fz += bfz;
                 if (log) {
                     bpe += 0.5 * (kb * (r - r0) * (r - r0));
-                    w0 += 0.5 * ddx * bfx;
-                    w1 += 0.5 * ddx * bfy;
-                    w2 += 0.5 * ddx * bfz;
-                    w3 += 0.5 * ddy * bfy;
-                    w4 += 0.5 * ddy * bfz;
-                    w5 += 0.5 * ddz * bfz;
+                    bw[0] += 0.5 * ddx * bfx;
+                    bw[1] += 0.5 * ddx * bfy;
+                    bw[2] += 0.5 * ddx * bfz;
+                    bw[3] += 0.5 * ddy * bfy;
+                    bw[4] += 0.5 * ddy * bfz;
+                    bw[5] += 0.5 * ddz * bfz;
                 }
             }
+            if (mode & kModeStoreForce) {
+                R.force[s] = fx;
+                R.force[N + s] = fy;
+                R.force[2 * N + s] = fz;
+            }
         }
-        if (force_only || (mode & kModeStoreForce)) {
-            R.force[slot] = fx;
-            R.force[N + slot] = fy;
-            R.force[2 * N + slot] = fz;
-        }
-        if (log) {
-            R.pe[slot] = pe;
-            R.bpe[slot] = bpe;
-            R.vir[slot] = w0;
-            R.vir[N + slot] = w1;
-            R.vir[2 * N + slot] = w2;
-            R.vir[3 * N + slot] = w3;
-            R.vir[4 * N + slot] = w4;
-            R.vir[5 * N + slot] = w5;
-        }
-        if (force_only) continue;
-
-        const int tt = L.sty[li] - 1;
+        const int tt = R.type[s] - 1;
         const double h = (0.5 * R.dt) * R.minv[tt];
-        double vxg = V[slot], vyg = V[N + slot], vzg = V[2 * N + slot];
+        double vxg = V[s], vyg = V[N + s], vzg = V[2 * N + s];
         // explicit FMAs: the two half-kicks round exactly as two separate
         // nve kernels do (fix/fix_nve.cu:54-56, :143-145)
         if (mode & kModeFinalKick) {
@@ -751,28 +899,27 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         }
         if (log) {
             const double m = R.mass[tt];
-            acc[0] += 0.5 * m * (vxg * vxg + vyg * vyg + vzg * vzg);
-            acc[1] += pe;
-            acc[2] += bpe;
-            acc[3] += m * vxg * vxg + w0;
-            acc[4] += m * vxg * vyg + w1;
-            acc[5] += m * vxg * vzg + w2;
-            acc[6] += m * vyg * vyg + w3;
-            acc[7] += m * vyg * vzg + w4;
-            acc[8] += m * vzg * vzg + w5;
+            acc[0] = 0.5 * m * (vxg * vxg + vyg * vyg + vzg * vzg);
+            acc[2] = bpe;
+            acc[3] = m * vxg * vxg + bw[0];
+            acc[4] = m * vxg * vyg + bw[1];
+            acc[5] = m * vxg * vzg + bw[2];
+            acc[6] = m * vyg * vyg + bw[3];
+            acc[7] = m * vyg * vzg + bw[4];
+            acc[8] = m * vzg * vzg + bw[5];
         }
         if (mode & kModeDrift) {
             vxg = fma(h, fx, vxg);
             vyg = fma(h, fy, vyg);
             vzg = fma(h, fz, vzg);
             int ix, iy, iz;
-            unpack_img(R.img[slot], ix, iy, iz);
+            unpack_img(R.img[s], ix, iy, iz);
             const int ix0 = ix, iy0 = iy, iz0 = iz;
             bool b2 = false;
             xg = wrap_axis(fma(R.dt, vxg, xg), R.lo[0], R.lo[0] + R.L[0], R.L[0], ix, b2);
             yg = wrap_axis(fma(R.dt, vyg, yg), R.lo[1], R.lo[1] + R.L[1], R.L[1], iy, b2);
             zg = wrap_axis(fma(R.dt, vzg, zg), R.lo[2], R.lo[2] + R.L[2], R.L[2], iz, b2);
-            if (ix != ix0 || iy != iy0 || iz != iz0) R.img[slot] = pack_img(ix, iy, iz);
+            if (ix != ix0 || iy != iy0 || iz != iz0) R.img[s] = pack_img(ix, iy, iz);
             if (b2 || !isfinite(vxg + vyg + vzg)) {
                 bad = true;
                 xg = R.lo[0];
@@ -780,22 +927,31 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
                 zg = R.lo[2];
             }
             const int key = cell_of(R, xg, yg, zg);
-            R.key[slot] = key;
-            R.rank[slot] = atomicAdd(&R.cell_count[key], 1);
+            double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
+            Xo[s] = xg;
+            Xo[N + s] = yg;
+            Xo[2 * N + s] = zg;
+            Vo[s] = vxg;
+            Vo[N + s] = vyg;
+            Vo[2 * N + s] = vzg;
+            R.key[s] = key;
+            R.rank[s] = atomicAdd(&R.cell_count[key], 1);
+        } else {
+            double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
+            Xo[s] = xg;
+            Xo[N + s] = yg;
+            Xo[2 * N + s] = zg;
+            Vo[s] = vxg;
+            Vo[N + s] = vyg;
+            Vo[2 * N + s] = vzg;
         }
-        double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
-        Xo[slot] = xg;
-        Xo[N + slot] = yg;
-        Xo[2 * N + slot] = zg;
-        Vo[slot] = vxg;
-        Vo[N + slot] = vyg;
-        Vo[2 * N + slot] = vzg;
     }
     if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
-    if (log && !force_only) {
-        double *out = L.red + 48;
-        block_sum<kThermoParts>(acc, L.red, out);
-        if (t < kThermoParts) R.thermo_part[(size_t)it.local * kThermoParts + t] = out[t];
+    if (log) {
+        long long *out = red + 48;
+        block_sum_fix<kThermoParts>(acc, red, out);
+        if (threadIdx.x < kThermoParts)
+            ((long long *)R.thermo_bead)[(size_t)it.local * kThermoParts + threadIdx.x] = out[threadIdx.x];
     }
 }
 
@@ -808,18 +964,39 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
                                                      const int *__restrict__ rep_list, int flip,
                                                      int advance, int thermo)
 {
-    __shared__ double red[16 * (kBlock / 64)];
-    __shared__ double tot[kThermoParts];
+    __shared__ long long red[64];
+    __shared__ long long itot[kThermoParts];
     const RepDev &R = reps[rep_list[blockIdx.x]];
     const long long step = R.state->step;
     if (thermo && R.log_every > 0 && (step % R.log_every) == 0) {
-        double acc[kThermoParts];
+        // exact integer sums of the fixed-point partials
+        long long acc[kThermoParts];
 #pragma unroll
-        for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
+        for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
+        const long long *tp = (const long long *)R.thermo_part, *tb = (const long long *)R.thermo_bead;
         for (int b = threadIdx.x; b < R.nbrick; b += kBlock)
 #pragma unroll
-            for (int k = 0; k < kThermoParts; ++k) acc[k] += R.thermo_part[(size_t)b * kThermoParts + k];
-        block_sum<kThermoParts>(acc, red, tot);
+            for (int k = 0; k < kThermoParts; ++k) acc[k] += tp[(size_t)b * kThermoParts + k];
+        const int nchunk = (R.N + kBlock - 1) / kBlock;
+        for (int b = threadIdx.x; b < nchunk; b += kBlock)
+#pragma unroll
+            for (int k = 0; k < kThermoParts; ++k) acc[k] += tb[(size_t)b * kThermoParts + k];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < kThermoParts; ++k) {
+            const long long sv = wave_sum_i64(acc[k]);
+            if (lane == 0) red[w * kThermoParts + k] = sv;
+        }
+        __syncthreads();
+        if (threadIdx.x < kThermoParts) {
+            long long sv = 0;
+            for (int ww = 0; ww < kBlock / 64; ++ww) sv += red[ww * kThermoParts + threadIdx.x];
+            itot[threadIdx.x] = sv;
+        }
+        __syncthreads();
+        double tot[kThermoParts];
+#pragma unroll
+        for (int k = 0; k < kThermoParts; ++k) tot[k] = (double)itot[k] / kThermoFix;
         if (threadIdx.x == 0) {
             const double vol = R.L[0] * R.L[1] * R.L[2];
             const int row = R.state->log_rows % R.thermo_cap;
@@ -886,7 +1063,10 @@ __global__ __launch_bounds__(kBlock) void k_bond_hook(const RepDev *__restrict__
     const int N = R.N;
     const double *X = R.x[R.state->cur];
     const int id = R.id[s];
-    double f[3] = {0, 0, 0}, w[6] = {0, 0, 0, 0, 0, 0}, bpe = 0.0;
+    // added to the pair force one bond at a time, in the order the fused
+    // integration kernel uses, so both paths round identically
+    double f[3] = {R.force[s], R.force[N + s], R.force[2 * N + s]};
+    double w[6] = {0, 0, 0, 0, 0, 0}, bpe = 0.0;
     const int b1 = R.bond_start[id];
     for (int bb = R.bond_start[id - 1]; bb < b1; ++bb) {
         const int ps = R.slot_of_id[R.bond_partner[bb] - 1];
@@ -909,7 +1089,7 @@ __global__ __launch_bounds__(kBlock) void k_bond_hook(const RepDev *__restrict__
         w[4] += 0.5 * d[1] * bf[2];
         w[5] += 0.5 * d[2] * bf[2];
     }
-    for (int k = 0; k < 3; ++k) R.force[k * N + s] += f[k];
+    for (int k = 0; k < 3; ++k) R.force[k * N + s] = f[k];
     if (log) {
         R.bpe[s] = bpe;
         for (int k = 0; k < 6; ++k) R.vir[k * N + s] += w[k];
@@ -1073,6 +1253,13 @@ __global__ __launch_bounds__(kBlock) void k_saru_eval(int n, const uint32_t *__r
     r64[k] = saru_uniform<double>(b);
 }
 
+template <typename real>
+__global__ void k_fix_to_double(double *p, long long n)
+{
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = (double)__double_as_longlong(p[i]) * FixScale<real>::inv;
+}
+
 __global__ void k_fill_i32(int *p, int v, long long n)
 {
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -1092,8 +1279,7 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / 
 
 size_t BrickLaunch::lds_bytes(int precision, int style) const
 {
-    (void)style;
-    BrickParams p{rmax, smax, kmax, ncol, ntypes};
+    BrickParams p{rmax, smax, omax, ncol, ntypes, qw, perbead, style};
     return precision == 1 ? lds_offsets<double>(p).total : lds_offsets<float>(p).total;
 }
 
@@ -1129,8 +1315,8 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.kmax, bl.ncol, bl.ntypes};
-    const size_t lds = bl.lds_bytes(precision, 2);
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.ncol, bl.ntypes, bl.qw, 0, 1};
+    const size_t lds = bl.lds_bytes(precision, 1);
     if (precision == 1)
         hipLaunchKernelGGL(k_density<double>, dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p);
     else
@@ -1140,7 +1326,7 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
                   int precision, int style, int mode, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.kmax, bl.ncol, bl.ntypes};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.ncol, bl.ntypes, bl.qw, bl.perbead, style};
     const size_t lds = bl.lds_bytes(precision, style);
     if (precision == 1) {
         if (style == 2)
@@ -1153,6 +1339,10 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
         else
             hipLaunchKernelGGL((k_brick<float, 1>), dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p, mode);
     }
+}
+void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_integrate, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, mode);
 }
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
                      int thermo, hipStream_t s)
@@ -1202,6 +1392,12 @@ void launch_saru_eval(int n, const uint32_t *t1, const uint32_t *t2, uint32_t se
                       uint32_t *bits, float *r32, double *r64, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_saru_eval, dim3(nblk(n)), dim3(kBlock), 0, s, n, t1, t2, seed, steps, bits, r32, r64);
+}
+void launch_fix_to_double(double *p, long long n, int precision, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (precision == 1) hipLaunchKernelGGL(k_fix_to_double<double>, dim3(nblk(n)), dim3(kBlock), 0, s, p, n);
+    else hipLaunchKernelGGL(k_fix_to_double<float>, dim3(nblk(n)), dim3(kBlock), 0, s, p, n);
 }
 void launch_fill_i32(int *p, int v, long long n, hipStream_t s)
 {

@@ -217,5 +217,5 @@ class System:
     def tuning(self):
         out = np.zeros(8, dtype=np.int32)
         self.ctx.check(self.lib.gdpd_get_tuning(self.ctx.h, self.sid, _ptr(out, C.c_int)))
-        keys = ["bx", "by", "bz", "smax", "kmax", "resort_every", "graph_steps", "nbrick"]
+        keys = ["bx", "by", "bz", "smax", "omax", "resort_every", "graph_steps", "nbrick"]
         return dict(zip(keys, out.tolist()))

@@ -99,3 +99,24 @@ def test_cpp_plugin_shells_hook_equals_fused():
     assert np.allclose(hook, fused, rtol=1e-11, atol=1e-11)
     final = [l for l in lines if l.startswith("final")][0].split()
     assert float(final[1]) == 0.0 and float(final[2]) == 0.0
+
+
+def test_bonded_hook_path_equals_fused_path(ctx):
+    cfg = configs.config_c2(n_chains=800, box=16.0)
+    n = 8
+    a = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=4)
+    a.run(n)
+    b = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=4)
+    b.pair_compute(0, log=True)
+    b.bond_compute(0, log=True)
+    for step in range(1, n + 1):
+        b.nve_initial()
+        b.pair_compute(step, log=False)
+        b.bond_compute(step, log=False)
+        b.nve_final()
+    pa, va = state(a)
+    pb, vb = state(b)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    assert np.array_equal(a.download()["force"], b.download()["force"])
+    a.destroy()
+    b.destroy()
