@@ -461,6 +461,33 @@ __device__ __forceinline__ double id_to_real(int id, double) { return __longlong
 __device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_longlong(v); }
 
+// Owned beads are grouped by brick column (rx, ry).  Interior columns
+// (rx >= 2 and ry >= 2) come first: their beads find no partners in the lower
+// neighbour runs except in z-halo cells, so waves of interior beads skip most of
+// those runs together.
+__device__ __forceinline__ void column_of(const BrickGeom &g, int q, int &rx, int &ry)
+{
+    const int ni = (g.ext[0] - 1) * (g.ext[1] - 1);
+    if (q < ni) {
+        rx = 2 + q / (g.ext[1] - 1);
+        ry = 2 + q % (g.ext[1] - 1);
+    } else if (q - ni < g.ext[1]) {
+        rx = 1;
+        ry = 1 + (q - ni);
+    } else {
+        rx = 2 + (q - ni - g.ext[1]);
+        ry = 1;
+    }
+}
+
+__device__ __forceinline__ int column_index(const BrickGeom &g, int rx, int ry)
+{
+    const int ni = (g.ext[0] - 1) * (g.ext[1] - 1);
+    if (rx >= 2 && ry >= 2) return (rx - 2) * (g.ext[1] - 1) + (ry - 2);
+    if (rx == 1) return ni + (ry - 1);
+    return ni + g.ext[1] + (rx - 2);
+}
+
 // Stage the brick + halo.  Returns (uniformly) the number of owned beads, or
 // -1 when the staged or owned beads exceed the LDS capacity.
 template <typename real>
@@ -504,7 +531,9 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             L.colpre[0] = 0;
             const int ncol = g.ext[0] * g.ext[1];
             for (int q = 0; q < ncol; ++q) {
-                const int c = ((1 + q / g.ext[1]) * g.RY + (1 + q % g.ext[1])) * g.RZ;
+                int rx, ry;
+                column_of(g, q, rx, ry);
+                const int c = (rx * g.RY + ry) * g.RZ;
                 L.colpre[q + 1] = L.colpre[q] + (L.c_start[c + g.ext[2] + 1] - L.c_start[c + 1]);
             }
         }
@@ -516,47 +545,87 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
 
     const int N = R.N;
     const double *X = R.x[cur], *V = R.v[cur];
-    for (int k = t; k < S; k += kBlock) {
+    // four consecutive staged beads per thread: one binary search, then all
+    // perm loads and all state loads in flight together
+    for (int base = 4 * t; base < S; base += 4 * kBlock) {
         int lo = 0, hi = g.nreg;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
-            if (L.c_start[mid] <= k) lo = mid;
+            if (L.c_start[mid] <= base) lo = mid;
             else hi = mid;
         }
-        const int rc = lo;
-        const int rz = rc % g.RZ, ry = (rc / g.RZ) % g.RY, rx = rc / (g.RZ * g.RY);
-        int slot = R.perm[L.gstart[rc] + (k - L.c_start[rc])];
-        if ((unsigned)slot >= (unsigned)N) slot = 0;
-        double shx, shy, shz;
-        wrap_cell(g.c0[0] - 1 + rx, R.nc[0], R.L[0], shx);
-        wrap_cell(g.c0[1] - 1 + ry, R.nc[1], R.L[1], shy);
-        wrap_cell(g.c0[2] - 1 + rz, R.nc[2], R.L[2], shz);
-        real4 p;
-        p.x = (real)((X[slot] + shx) - g.org[0]);
-        p.y = (real)((X[N + slot] + shy) - g.org[1]);
-        p.z = (real)((X[2 * N + slot] + shz) - g.org[2]);
-        const bool inner = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
-        // type in w, negative for halo beads
-        p.w = type_to_real<real>(inner ? (int)R.type[slot] : -(int)R.type[slot]);
-        L.pos[k] = p;
-        if (need_vel) {
-            real4 q;
-            q.x = (real)V[slot];
-            q.y = (real)V[N + slot];
-            q.z = (real)V[2 * N + slot];
-            q.w = id_to_real(R.id[slot], (real)0);
-            L.vel[k] = q;
+        int rcs[4], gp[4], sl[4];
+        int rc = lo;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = base + u;
+            if (k < S) {
+                while (L.c_start[rc + 1] <= k) ++rc;
+                rcs[u] = rc;
+                gp[u] = L.gstart[rc] + (k - L.c_start[rc]);
+            } else {
+                rcs[u] = rc;
+                gp[u] = -1;
+            }
         }
-        if (need_rho) L.rh[k] = (real)R.rho[slot];
-        unsigned short ow = 0xFFFF;
-        if (inner) {
-            const int q = (rx - 1) * g.ext[1] + (ry - 1);
-            const int o = L.colpre[q] + (k - L.c_start[(rx * g.RY + ry) * g.RZ + 1]);
-            ow = (unsigned short)o;
-            L.oslot[o] = slot;
-            L.oinfo[o] = k | (rc << 16);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int slot = gp[u] >= 0 ? R.perm[gp[u]] : 0;
+            sl[u] = ((unsigned)slot < (unsigned)N) ? slot : 0;
         }
-        L.own[k] = ow;
+        double xs[4], ys[4], zs[4], vxs[4], vys[4], vzs[4];
+        int ids[4], tys[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int sv = sl[u];
+            xs[u] = X[sv];
+            ys[u] = X[N + sv];
+            zs[u] = X[2 * N + sv];
+            tys[u] = R.type[sv];
+            if (need_vel) {
+                vxs[u] = V[sv];
+                vys[u] = V[N + sv];
+                vzs[u] = V[2 * N + sv];
+                ids[u] = R.id[sv];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = base + u;
+            if (k >= S) continue;
+            const int rc2 = rcs[u];
+            const int rz = rc2 % g.RZ, ry = (rc2 / g.RZ) % g.RY, rx = rc2 / (g.RZ * g.RY);
+            double shx, shy, shz;
+            wrap_cell(g.c0[0] - 1 + rx, R.nc[0], R.L[0], shx);
+            wrap_cell(g.c0[1] - 1 + ry, R.nc[1], R.L[1], shy);
+            wrap_cell(g.c0[2] - 1 + rz, R.nc[2], R.L[2], shz);
+            const bool inner = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
+            real4 p;
+            p.x = (real)((xs[u] + shx) - g.org[0]);
+            p.y = (real)((ys[u] + shy) - g.org[1]);
+            p.z = (real)((zs[u] + shz) - g.org[2]);
+            // type in w, negative for halo beads
+            p.w = type_to_real<real>(inner ? tys[u] : -tys[u]);
+            L.pos[k] = p;
+            if (need_vel) {
+                real4 q;
+                q.x = (real)vxs[u];
+                q.y = (real)vys[u];
+                q.z = (real)vzs[u];
+                q.w = id_to_real(ids[u], (real)0);
+                L.vel[k] = q;
+            }
+            if (need_rho) L.rh[k] = (real)R.rho[sl[u]];
+            unsigned short ow = 0xFFFF;
+            if (inner) {
+                const int qc = column_index(g, rx, ry);
+                const int o = L.colpre[qc] + (k - L.c_start[(rx * g.RY + ry) * g.RZ + 1]);
+                ow = (unsigned short)o;
+                L.oslot[o] = sl[u];
+                L.oinfo[o] = k | (rc2 << 16);
+            }
+            L.own[k] = ow;
+        }
     }
     for (int k = t; k < R.ntypes * R.ntypes; k += kBlock) L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
     for (int k = t; k < 3 * O; k += kBlock) L.facc[k] = 0;
@@ -722,28 +791,61 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
         for (int r = 0; r < 9; ++r) {
             int js = 0, len = 0;
             if (valid) {
-                const int cb = ((rx + r / 3 - 1) * g.RY + (ry + r % 3 - 1)) * g.RZ + rz - 1;
-                js = L.c_start[cb];
-                len = L.c_start[cb + 3] - js;
+                const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
+                const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
+                const int A = L.c_start[cb], D = L.c_start[cb + 3];
+                const bool zlo = (rz == 1), zhi = (rz + 1 == g.RZ - 1);
+                if (r < 4) {
+                    // every cell here precedes li's cell: only halo beads can
+                    // pair (whole halo columns, or the z-halo end cells)
+                    const bool xyhalo = cx == 0 || cy == 0 || cy == g.RY - 1;
+                    if (xyhalo || (zlo && zhi)) {
+                        js = A;
+                        len = D - A;
+                    } else if (zlo) {
+                        js = A;
+                        len = L.c_start[cb + 1] - A;
+                    } else if (zhi) {
+                        js = L.c_start[cb + 2];
+                        len = D - js;
+                    }
+                } else if (r == 4) {
+                    js = zlo ? A : li + 1;
+                    len = D - js;
+                } else {
+                    js = A;
+                    len = D - A;
+                }
             }
             const int maxlen = wave_max(len);
-            for (int k = 0; k < maxlen; ++k) {
-                if (qn > qw - 64) {
+            for (int k = 0; k < maxlen; k += 2) {
+                if (qn > qw - 128) {
                     eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
                     qn = 0;
                 }
-                bool pass = false;
-                const int lj = js + k;
+                bool pa = false, pb = false;
+                const int la = js + k, lb = js + k + 1;
+                real4 pj0, pj1;
+                if (k < len) pj0 = L.pos[la];
+                if (k + 1 < len) pj1 = L.pos[lb];
                 if (k < len) {
-                    const real4 pj = L.pos[lj];
-                    const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+                    const real dx = pi.x - pj0.x, dy = pi.y - pj0.y, dz = pi.z - pj0.z;
                     const real r2 = dx * dx + dy * dy + dz * dz;
                     const bool inr = DENSITY ? (r2 < cut2) : (r2 <= cut2 && r2 >= (real)1.0e-20);
-                    pass = inr && (lj > li || pj.w < (real)0);
+                    pa = inr && (la > li || pj0.w < (real)0);
                 }
-                const unsigned long long m = __ballot(pass);
-                if (pass) q[qn + lanes_below(m)] = (unsigned)li | ((unsigned)lj << 16);
-                qn += __popcll(m);
+                if (k + 1 < len) {
+                    const real dx = pi.x - pj1.x, dy = pi.y - pj1.y, dz = pi.z - pj1.z;
+                    const real r2 = dx * dx + dy * dy + dz * dz;
+                    const bool inr = DENSITY ? (r2 < cut2) : (r2 <= cut2 && r2 >= (real)1.0e-20);
+                    pb = inr && (lb > li || pj1.w < (real)0);
+                }
+                const unsigned long long ma = __ballot(pa);
+                if (pa) q[qn + lanes_below(ma)] = (unsigned)li | ((unsigned)la << 16);
+                qn += __popcll(ma);
+                const unsigned long long mb = __ballot(pb);
+                if (pb) q[qn + lanes_below(mb)] = (unsigned)li | ((unsigned)lb << 16);
+                qn += __popcll(mb);
             }
         }
     }
