@@ -7,8 +7,16 @@ arithmetic with FP64 accumulators and FP64 state.  A "step" is one whole
 velocity-Verlet step: pair force + final half-kick + half-kick + drift + re-binning.
 
 Prints one JSON line with the metric, the roofline of the dominant kernel (the
-fused force+integrate kernel, timed with HIP events on the stream it runs on)
-and a CPU baseline (the OpenMP oracle port on a bounded sample).
+brick pair-force kernel k_brick, timed with HIP events on the library stream it
+runs on) and a CPU baseline (the OpenMP oracle port on a bounded sample).
+
+Algorithmic bytes (compulsory HBM traffic, FP64 state, each array touched once):
+  force pass (k_brick)   read x 24 + v 24 + id 4 + type 4, write f 24 = 80 B/bead
+                         (+ rho 8 read for mDPD)
+  integrate (k_integrate) read f, v, x (72), write v, x (48)       = 120 B/bead
+  density pass (mDPD)    read x 24 + type 4, write rho 8           =  36 B/bead
+  bonds                  12 B/bond
+so a DPD step is 200 B/bead and an mDPD step 244 B/bead.
 """
 import argparse
 import json
@@ -21,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_BEAD_STEP = {"dpd": 200.0, "mdpd": 244.0}
+FORCE_PASS_BYTES = {"dpd": 80.0, "mdpd": 88.0}
 
 
 def log(msg):
@@ -111,8 +120,10 @@ def main():
     N = cfg["N"]
     value = N * args.steps / wall
     per = BYTES_PER_BEAD_STEP[cfg["style"]] + (12.0 * len(cfg["bonds"]) / N if "bonds" in cfg else 0.0)
+    per_kernel = FORCE_PASS_BYTES[cfg["style"]]
     k_avg = k_ms / max(k_n, 1) / 1e3
-    achieved = per * N / k_avg / 1e9
+    achieved = per_kernel * N / k_avg / 1e9
+    step_gbs = per * value / 1e9
     workload = {"c0": "dpd_fluid_24k", "c1": "mdpd_slab_8k", "c2": "diblock_melt_375k",
                 "c3": "dpd_fluid_10M"}[args.config]
     out = {
@@ -134,8 +145,10 @@ def main():
                    "graph_steps": 0},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
-                     "kernel": "k_brick (fused pair force + integrate)", "kernel_ms": k_avg * 1e3,
-                     "bytes_per_bead_step": per},
+                     "kernel": "k_brick (pair force pass)", "kernel_ms": k_avg * 1e3,
+                     "bytes_per_bead": per_kernel, "launches_timed": k_n,
+                     "step": {"bytes_per_bead_step": per, "achieved": step_gbs,
+                              "frac": step_gbs / HBM_PEAK_GBS}},
     }
     ctx.close()
     if not args.no_cpu:

@@ -772,6 +772,7 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int qw = bp.qw;
     unsigned *q = L.queue + w * qw;
+    const int last = L.c_start[g.nreg] - 1;
     int qn = 0;
     for (int chunk = w; chunk * 64 < O; chunk += kBlock / 64) {
         const int o = chunk * 64 + lane;
@@ -823,29 +824,25 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                     eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
                     qn = 0;
                 }
-                bool pa = false, pb = false;
-                const int la = js + k, lb = js + k + 1;
-                real4 pj0, pj1;
-                if (k < len) pj0 = L.pos[la];
-                if (k + 1 < len) pj1 = L.pos[lb];
-                if (k < len) {
-                    const real dx = pi.x - pj0.x, dy = pi.y - pj0.y, dz = pi.z - pj0.z;
-                    const real r2 = dx * dx + dy * dy + dz * dz;
-                    const bool inr = DENSITY ? (r2 < cut2) : (r2 <= cut2 && r2 >= (real)1.0e-20);
-                    pa = inr && (la > li || pj0.w < (real)0);
-                }
-                if (k + 1 < len) {
-                    const real dx = pi.x - pj1.x, dy = pi.y - pj1.y, dz = pi.z - pj1.z;
-                    const real r2 = dx * dx + dy * dy + dz * dz;
-                    const bool inr = DENSITY ? (r2 < cut2) : (r2 <= cut2 && r2 >= (real)1.0e-20);
-                    pb = inr && (lb > li || pj1.w < (real)0);
-                }
+                // loads clamped into the staged range so they need no mask;
+                // lanes past their run length are dropped by the k < len test
+                const int la = js + k, lb = la + 1;
+                const real4 pj0 = L.pos[min(la, last)];
+                const real4 pj1 = L.pos[min(lb, last)];
+                const real dx0 = pi.x - pj0.x, dy0 = pi.y - pj0.y, dz0 = pi.z - pj0.z;
+                const real dx1 = pi.x - pj1.x, dy1 = pi.y - pj1.y, dz1 = pi.z - pj1.z;
+                const real r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+                const real r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+                const bool in0 = DENSITY ? (r20 < cut2) : (r20 <= cut2 && r20 >= (real)1.0e-20);
+                const bool in1 = DENSITY ? (r21 < cut2) : (r21 <= cut2 && r21 >= (real)1.0e-20);
+                const bool pa = k < len && in0 && (la > li || pj0.w < (real)0);
+                const bool pb = k + 1 < len && in1 && (lb > li || pj1.w < (real)0);
                 const unsigned long long ma = __ballot(pa);
-                if (pa) q[qn + lanes_below(ma)] = (unsigned)li | ((unsigned)la << 16);
-                qn += __popcll(ma);
                 const unsigned long long mb = __ballot(pb);
-                if (pb) q[qn + lanes_below(mb)] = (unsigned)li | ((unsigned)lb << 16);
-                qn += __popcll(mb);
+                const int na = __popcll(ma);
+                if (pa) q[qn + lanes_below(ma)] = (unsigned)li | ((unsigned)la << 16);
+                if (pb) q[qn + na + lanes_below(mb)] = (unsigned)li | ((unsigned)lb << 16);
+                qn = __builtin_amdgcn_readfirstlane(qn + na + __popcll(mb));
             }
         }
     }
