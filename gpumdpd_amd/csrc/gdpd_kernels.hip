@@ -354,7 +354,7 @@ template <typename real> struct BrickLds {
     real4 *pos, *vel;          // xyz + type, vxyz + id bits
     real *rh;                  // staged density (mDPD force pass)
     unsigned short *own;       // owned index, or 0xFFFF for halo beads
-    int *c_start, *c_src, *gstart, *colpre, *wbuf, *oslot, *oinfo;
+    int *c_start, *cinfo, *gstart, *colpre, *wbuf, *oslot, *oinfo;
     long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
 };
@@ -364,7 +364,7 @@ __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t
 // LDS layout as byte offsets; pointers are rebuilt inline from the __shared__
 // base so they stay in the LDS address space.
 struct LdsOffsets {
-    unsigned coef, red, pos, vel, rh, own, c_start, c_src, gstart, colpre, wbuf, oslot, oinfo, facc, queue, total;
+    unsigned coef, red, pos, vel, rh, own, c_start, cinfo, gstart, colpre, wbuf, oslot, oinfo, facc, queue, total;
 };
 
 template <typename real>
@@ -377,12 +377,12 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     off = align16(off + (size_t)(bytes));
     GDPD_TAKE(coef, sizeof(PairCoef<real>) * p.ntypes * p.ntypes)
     GDPD_TAKE(red, sizeof(double) * 64)
-    GDPD_TAKE(pos, sizeof(real) * 4 * p.smax)
+    GDPD_TAKE(pos, sizeof(real) * 4 * (p.smax + 1))
     GDPD_TAKE(vel, sizeof(real) * 4 * p.smax)
     GDPD_TAKE(rh, sizeof(real) * (p.style == 2 ? p.smax : 1))
     GDPD_TAKE(own, sizeof(unsigned short) * p.smax)
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
-    GDPD_TAKE(c_src, sizeof(int) * p.rmax)
+    GDPD_TAKE(cinfo, sizeof(int) * p.rmax)
     GDPD_TAKE(gstart, sizeof(int) * p.rmax)
     GDPD_TAKE(colpre, sizeof(int) * (p.ncol + 1))
     GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
@@ -407,7 +407,7 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.rh = (real *)(smem + o.rh);
     L.own = (unsigned short *)(smem + o.own);
     L.c_start = (int *)(smem + o.c_start);
-    L.c_src = (int *)(smem + o.c_src);
+    L.cinfo = (int *)(smem + o.cinfo);
     L.gstart = (int *)(smem + o.gstart);
     L.colpre = (int *)(smem + o.colpre);
     L.wbuf = (int *)(smem + o.wbuf);
@@ -441,17 +441,19 @@ __device__ __forceinline__ BrickGeom brick_geom(const RepDev &R, int b)
     return g;
 }
 
-__device__ __forceinline__ int wrap_cell(int c, int n, double L, double &shift)
+// Periodic wrap of a cell coordinate in [-n, 2n): returns the cell and sets
+// code to the image shift + 1 (0: one box below, 1: none, 2: one box above).
+__device__ __forceinline__ int wrap_code(int c, int n, int &code)
 {
     if (c < 0) {
-        shift = -L;
+        code = 0;
         return c + n;
     }
     if (c >= n) {
-        shift = L;
+        code = 2;
         return c - n;
     }
-    shift = 0.0;
+    code = 1;
     return c;
 }
 
@@ -498,13 +500,13 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     const int t = threadIdx.x;
     for (int rc = t; rc < g.nreg; rc += kBlock) {
         const int rz = rc % g.RZ, ry = (rc / g.RZ) % g.RY, rx = rc / (g.RZ * g.RY);
-        double sh;
-        const int gx = wrap_cell(g.c0[0] - 1 + rx, R.nc[0], R.L[0], sh);
-        const int gy = wrap_cell(g.c0[1] - 1 + ry, R.nc[1], R.L[1], sh);
-        const int gz = wrap_cell(g.c0[2] - 1 + rz, R.nc[2], R.L[2], sh);
+        int wx, wy, wz;
+        const int gx = wrap_code(g.c0[0] - 1 + rx, R.nc[0], wx);
+        const int gy = wrap_code(g.c0[1] - 1 + ry, R.nc[1], wy);
+        const int gz = wrap_code(g.c0[2] - 1 + rz, R.nc[2], wz);
         const int gc = (gx * R.nc[1] + gy) * R.nc[2] + gz;
         const int a = R.cell_start[gc], b = R.cell_start[gc + 1];
-        L.c_src[rc] = gc;
+        L.cinfo[rc] = rx | (ry << 8) | (rz << 16) | (wx << 24) | (wy << 26) | (wz << 28);
         L.gstart[rc] = a;
         L.c_start[rc + 1] = b - a;
     }
@@ -545,28 +547,21 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
 
     const int N = R.N;
     const double *X = R.x[cur], *V = R.v[cur];
-    // four consecutive staged beads per thread: one binary search, then all
-    // perm loads and all state loads in flight together
-    for (int base = 4 * t; base < S; base += 4 * kBlock) {
-        int lo = 0, hi = g.nreg;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (L.c_start[mid] <= base) lo = mid;
-            else hi = mid;
-        }
+    // four staged beads per thread, 256 apart so each load instruction stays
+    // coalesced; all perm loads and then all state loads are issued together
+    for (int base = t; base < S; base += 4 * kBlock) {
         int rcs[4], gp[4], sl[4];
-        int rc = lo;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int k = base + u;
-            if (k < S) {
-                while (L.c_start[rc + 1] <= k) ++rc;
-                rcs[u] = rc;
-                gp[u] = L.gstart[rc] + (k - L.c_start[rc]);
-            } else {
-                rcs[u] = rc;
-                gp[u] = -1;
+            const int k = base + u * kBlock;
+            int lo = 0, hi = g.nreg;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (L.c_start[mid] <= k) lo = mid;
+                else hi = mid;
             }
+            rcs[u] = lo;
+            gp[u] = k < S ? L.gstart[lo] + (k - L.c_start[lo]) : -1;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -591,14 +586,14 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int k = base + u;
+            const int k = base + u * kBlock;
             if (k >= S) continue;
             const int rc2 = rcs[u];
-            const int rz = rc2 % g.RZ, ry = (rc2 / g.RZ) % g.RY, rx = rc2 / (g.RZ * g.RY);
-            double shx, shy, shz;
-            wrap_cell(g.c0[0] - 1 + rx, R.nc[0], R.L[0], shx);
-            wrap_cell(g.c0[1] - 1 + ry, R.nc[1], R.L[1], shy);
-            wrap_cell(g.c0[2] - 1 + rz, R.nc[2], R.L[2], shz);
+            const int ci = L.cinfo[rc2];
+            const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
+            const double shx = (double)(((ci >> 24) & 3) - 1) * R.L[0];
+            const double shy = (double)(((ci >> 26) & 3) - 1) * R.L[1];
+            const double shz = (double)(((ci >> 28) & 3) - 1) * R.L[2];
             const bool inner = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
             real4 p;
             p.x = (real)((xs[u] + shx) - g.org[0]);
@@ -639,6 +634,21 @@ template <> __device__ __forceinline__ double ratio<double>(double a, double b, 
 
 // fixed-point scale of the per-bead sums: 2^32 for FP32 pair arithmetic,
 // 2^40 (|sum| < 8.3e6) for FP64
+// r and 1/r of a pair distance.  FP32 mode uses the hardware reciprocal
+// square root (about 1 ulp, inside the FP32-mode force tolerance); FP64 mode
+// the correctly rounded square root and division.
+__device__ __forceinline__ void root_and_inverse(float r2, float &r, float &rinv)
+{
+    rinv = __builtin_amdgcn_rsqf(r2);
+    r = r2 * rinv;
+}
+
+__device__ __forceinline__ void root_and_inverse(double r2, double &r, double &rinv)
+{
+    r = sqrt(r2);
+    rinv = 1.0 / r;
+}
+
 template <typename real> struct FixScale;
 template <> struct FixScale<float> { static constexpr double s = 4294967296.0, inv = 1.0 / 4294967296.0; };
 template <> struct FixScale<double> { static constexpr double s = 1099511627776.0, inv = 1.0 / 1099511627776.0; };
@@ -651,6 +661,12 @@ template <typename real> __device__ __forceinline__ long long to_fix(double v)
 __device__ __forceinline__ void lds_add(long long *p, long long v)
 {
     atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+
+__device__ __forceinline__ void lds_sub(long long *p, long long v)
+{
+    __hip_atomic_fetch_sub((unsigned long long *)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ int wave_max(int v)
@@ -682,8 +698,8 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const real r2 = dx * dx + dy * dy + dz * dz;
         const PairCoef<real> &C = L.coef[((int)fabs(pi.w) - 1) * nt + ((int)fabs(pj.w) - 1)];
-        const real r = sqrt(r2);
         if (DENSITY) {
+            const real r = sqrt(r2);
             // force/pair_mdpd.cu:266-289
             real wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
             wd = wd > (real)0 ? wd : (real)0;
@@ -692,9 +708,11 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             if (jown) lds_add(&L.facc[oj], w);
             continue;
         }
-        if (r2 > C.rc2) continue;
+        // force/pair_dpd.cu: pairs inside rc with r >= 1e-10
+        if (r2 > C.rc2 || r2 < (real)1.0e-20) continue;
         const real4 vi = L.vel[li], vj = L.vel[lj];
-        const real rinv = (real)1 / r;
+        real r, rinv;
+        root_and_inverse(r2, r, rinv);
         const real dot = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
         const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
         const real wr = wc;
@@ -721,9 +739,9 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         lds_add(&L.facc[3 * oi + 1], qy);
         lds_add(&L.facc[3 * oi + 2], qz);
         if (jown) {
-            lds_add(&L.facc[3 * oj + 0], -qx);
-            lds_add(&L.facc[3 * oj + 1], -qy);
-            lds_add(&L.facc[3 * oj + 2], -qz);
+            lds_sub(&L.facc[3 * oj + 0], qx);
+            lds_sub(&L.facc[3 * oj + 1], qy);
+            lds_sub(&L.facc[3 * oj + 2], qz);
         }
         if (log) {
             // per-bead shares as in force/pair_dpd.cu:366-398 and pair_mdpd.cu:479-516
@@ -759,10 +777,12 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
 }
 
 // Generation + evaluation.  Wave w takes owned beads in chunks of 64 (chunks
-// w, w+4, ...).  Each lane scans its bead's 27 neighbour cells as 9 z-runs of 3
-// cells (contiguous in LDS); a candidate j is kept when it is in range and
-// either lies after the bead in LDS order or is a halo bead, so a pair of owned
-// beads is found once and a pair with a halo bead once per brick.
+// w, w+4, ...).  Each lane scans its bead's 27 neighbour cells as contiguous
+// LDS segments that hold only the partners j the bead owns the pair with: j
+// after the bead in LDS order, or a halo bead.  A pair of owned beads is thus
+// found once and a pair with a halo bead once per brick.  In-range candidates
+// are appended to a per-wave queue (ballot + mbcnt) that is evaluated 64 pairs
+// at a time.
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
                                             const BrickParams &bp, int O, int nt, real cut2, uint32_t seed_step,
@@ -772,7 +792,7 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int qw = bp.qw;
     unsigned *q = L.queue + w * qw;
-    const int last = L.c_start[g.nreg] - 1;
+    const bool thin = g.ext[2] == 1;
     int qn = 0;
     for (int chunk = w; chunk * 64 < O; chunk += kBlock / 64) {
         const int o = chunk * 64 + lane;
@@ -783,65 +803,86 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
         if (valid) {
             const int info = L.oinfo[o];
             li = info & 0xFFFF;
-            const int rc = info >> 16;
-            rz = rc % g.RZ;
-            ry = (rc / g.RZ) % g.RY;
-            rx = rc / (g.RZ * g.RY);
+            const int ci = L.cinfo[info >> 16];
+            rx = ci & 255;
+            ry = (ci >> 8) & 255;
+            rz = (ci >> 16) & 255;
             pi = L.pos[li];
         }
-        for (int r = 0; r < 9; ++r) {
-            int js = 0, len = 0;
+        // Ten candidate segments per bead, each a contiguous LDS range that
+        // holds only partners the bead owns the pair with:
+        //   s 0..3  z-runs of the four xy-neighbour columns that precede the
+        //           bead's column: whole run when the column is halo, else only
+        //           its halo end cell(s);
+        //   s 4     the bead's own column, bottom halo cell (rz == 1 only);
+        //   s 5     the bead's own column after the bead;
+        //   s 6..9  z-runs of the four following columns.
+        // Only a brick one cell thick in z (both z-ends halo) keeps its
+        // precedes-or-halo test, in segments 0..3.
+        for (int s = 0; s < 10; ++s) {
+            int js = 0, je = 0;
             if (valid) {
+                const int r = s < 5 ? s : s - 1;
                 const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
                 const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
                 const int A = L.c_start[cb], D = L.c_start[cb + 3];
                 const bool zlo = (rz == 1), zhi = (rz + 1 == g.RZ - 1);
-                if (r < 4) {
-                    // every cell here precedes li's cell: only halo beads can
-                    // pair (whole halo columns, or the z-halo end cells)
+                if (s < 4) {
                     const bool xyhalo = cx == 0 || cy == 0 || cy == g.RY - 1;
                     if (xyhalo || (zlo && zhi)) {
                         js = A;
-                        len = D - A;
+                        je = D;
                     } else if (zlo) {
                         js = A;
-                        len = L.c_start[cb + 1] - A;
+                        je = L.c_start[cb + 1];
                     } else if (zhi) {
                         js = L.c_start[cb + 2];
-                        len = D - js;
+                        je = D;
                     }
-                } else if (r == 4) {
-                    js = zlo ? A : li + 1;
-                    len = D - js;
+                } else if (s == 4) {
+                    if (zlo) {
+                        js = A;
+                        je = L.c_start[cb + 1];
+                    }
+                } else if (s == 5) {
+                    js = li + 1;
+                    je = D;
                 } else {
                     js = A;
-                    len = D - A;
+                    je = D;
                 }
             }
+            const int len = je - js;
+            if (__ballot(len > 0) == 0) continue;
             const int maxlen = wave_max(len);
+            const bool test_order = thin && s < 4;
             for (int k = 0; k < maxlen; k += 2) {
                 if (qn > qw - 128) {
                     eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
                     qn = 0;
                 }
-                // loads clamped into the staged range so they need no mask;
-                // lanes past their run length are dropped by the k < len test
-                const int la = js + k, lb = la + 1;
-                const real4 pj0 = L.pos[min(la, last)];
-                const real4 pj1 = L.pos[min(lb, last)];
+                // the pair of loads starts at most at smax - 1 (pos holds smax + 1
+                // entries); lanes past their segment are dropped by the k < len test
+                const int la = js + k;
+                const int lc = min(la, bp.smax - 1);
+                const real4 pj0 = L.pos[lc];
+                const real4 pj1 = L.pos[lc + 1];
                 const real dx0 = pi.x - pj0.x, dy0 = pi.y - pj0.y, dz0 = pi.z - pj0.z;
                 const real dx1 = pi.x - pj1.x, dy1 = pi.y - pj1.y, dz1 = pi.z - pj1.z;
                 const real r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
                 const real r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-                const bool in0 = DENSITY ? (r20 < cut2) : (r20 <= cut2 && r20 >= (real)1.0e-20);
-                const bool in1 = DENSITY ? (r21 < cut2) : (r21 <= cut2 && r21 >= (real)1.0e-20);
-                const bool pa = k < len && in0 && (la > li || pj0.w < (real)0);
-                const bool pb = k + 1 < len && in1 && (lb > li || pj1.w < (real)0);
+                bool pa = k < len && (DENSITY ? (r20 < cut2) : (r20 <= cut2));
+                bool pb = k + 1 < len && (DENSITY ? (r21 < cut2) : (r21 <= cut2));
+                if (test_order) {
+                    pa = pa && (la > li || pj0.w < (real)0);
+                    pb = pb && (la + 1 > li || pj1.w < (real)0);
+                }
                 const unsigned long long ma = __ballot(pa);
                 const unsigned long long mb = __ballot(pb);
                 const int na = __popcll(ma);
-                if (pa) q[qn + lanes_below(ma)] = (unsigned)li | ((unsigned)la << 16);
-                if (pb) q[qn + na + lanes_below(mb)] = (unsigned)li | ((unsigned)lb << 16);
+                const unsigned ent = (unsigned)li | ((unsigned)la << 16);
+                if (pa) q[qn + lanes_below(ma)] = ent;
+                if (pb) q[qn + na + lanes_below(mb)] = ent + 0x10000u;
                 qn = __builtin_amdgcn_readfirstlane(qn + na + __popcll(mb));
             }
         }
