@@ -6,6 +6,10 @@ Workload (default): the 10,077,696-bead DPD fluid (216^3 jittered lattice in a
 arithmetic with FP64 accumulators and FP64 state.  A "step" is one whole
 velocity-Verlet step: pair force + final half-kick + half-kick + drift + re-binning.
 
+The served-mix half of the metric (512 Poisson replica requests, continuous
+batching, p50/p99 step latency, against the plain one-at-a-time path) is the
+"serving" object of the same line; --no-serve skips it.
+
 Prints one JSON line with the metric, the roofline of the dominant kernel (the
 brick pair-force kernel k_brick, timed with HIP events on the library stream it
 runs on) and a CPU baseline (the OpenMP oracle port on a bounded sample).
@@ -64,6 +68,38 @@ def cpu_baseline(seconds=12.0, n_target=1_000_000):
                       f"half neighbour list rebuilt on the skin criterion, {threads} OpenMP threads"}
 
 
+def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
+    """Served mixed-replica stream (512 Poisson requests at 200/s, 70% DPD at
+    rho=3 / 30% mDPD at rho=6, 2,000-200,000 beads log-uniform, 2,000 steps,
+    FP32 force) with continuous batching, then the plain path (one replica at a
+    time, arrival order) on every plain_every-th request."""
+    import gpumdpd_amd as g
+    from gpumdpd_amd import serve, tune
+    t = time.perf_counter()
+    tuned = {st: tune.tune_brick(ctx, st, g.PREC_F32) for st in ("dpd", "mdpd")}
+    log(f"bench: tuned bricks {tuned} in {time.perf_counter() - t:.1f}s")
+    reqs = serve.make_requests(5, n=n_req, steps=steps)
+    serve.prepare_inputs(reqs)
+    log(f"bench: serving {n_req} requests, {sum(r.n for r in reqs)} beads in total")
+    m = serve.serve(ctx, reqs, max_slots=slots, chunk=chunk, tuned=tuned)
+    m.pop("kept")
+    log(f"bench: served in {m['makespan_s']:.1f}s")
+    sample = reqs[::plain_every]
+    p = serve.serve_plain(ctx, sample, chunk=chunk, tuned=tuned, realtime=False)
+    p.pop("kept")
+    log(f"bench: plain sample of {len(sample)} in {p['makespan_s']:.1f}s")
+    return {"metric": "served-mix replica-steps/s", "value": m["replica_steps_per_s"], "unit": "replica-steps/s",
+            "step_latency_p50_ms": m["step_latency_p50_ms"], "step_latency_p99_ms": m["step_latency_p99_ms"],
+            "request_latency_p50_s": m["request_latency_p50_s"], "request_latency_p99_s": m["request_latency_p99_s"],
+            "bead_steps_per_s": m["bead_steps_per_s"], "makespan_s": m["makespan_s"], "requests": n_req,
+            "steps_per_request": steps, "slots": slots, "chunk_steps": chunk, "tuned_bricks": tuned,
+            "plain": {"replica_steps_per_s": p["replica_steps_per_s"], "bead_steps_per_s": p["bead_steps_per_s"],
+                      "step_latency_p50_ms": p["step_latency_p50_ms"], "step_latency_p99_ms": p["step_latency_p99_ms"],
+                      "sample": f"every {plain_every}th request ({len(sample)} of {n_req}), one at a time, "
+                                "back to back"},
+            "speedup_vs_plain": m["replica_steps_per_s"] / p["replica_steps_per_s"]}
+
+
 def load_traffic(workload):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -84,6 +120,12 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-steps", type=int, default=2000)
+    ap.add_argument("--serve-slots", type=int, default=128)
+    ap.add_argument("--serve-chunk", type=int, default=16)
+    ap.add_argument("--plain-every", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -127,7 +169,8 @@ def main():
     workload = {"c0": "dpd_fluid_24k", "c1": "mdpd_slab_8k", "c2": "diblock_melt_375k",
                 "c3": "dpd_fluid_10M"}[args.config]
     out = {
-        "metric": "bead-steps/s at 10M-bead DPD (FP32 force)" if args.config == "c3" else f"bead-steps/s ({workload})",
+        "metric": ("bead-steps/s at 10M-bead DPD (FP32 force); served-mix replica-steps/s + p99"
+                   if args.config == "c3" else f"bead-steps/s ({workload})"),
         "value": value,
         "unit": "bead-steps/s",
         "n_gpus": 1,
@@ -150,6 +193,13 @@ def main():
                      "step": {"bytes_per_bead_step": per, "achieved": step_gbs,
                               "frac": step_gbs / HBM_PEAK_GBS}},
     }
+    s.destroy()
+    if not args.no_serve:
+        try:
+            out["serving"] = serving_leg(ctx, args.serve_requests, args.serve_steps, args.serve_slots,
+                                         args.serve_chunk, args.plain_every)
+        except Exception as e:  # reported, not fatal to the headline number
+            out["serving"] = {"error": repr(e)}
     ctx.close()
     if not args.no_cpu:
         log("bench: CPU baseline ...")

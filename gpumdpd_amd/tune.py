@@ -1,0 +1,72 @@
+"""Brick-shape autotuner.
+
+The force kernels stage one brick of bx*by*bz cells (plus a one-cell halo) per
+workgroup, so the brick shape is the kernel's tile.  The best tile depends on
+the bead density per cell (LDS capacity, occupancy) and on the pair style.
+The tuner times the force launches of a few candidate shapes on a sample system
+of the given class and keeps the fastest; results are cached per class for the
+life of the process.
+
+The other knobs of the reference's neighbour machinery do not exist in this
+design: cells are rebuilt every step (no Verlet skin, no rebuild interval), and
+the cell edge is the pair cutoff.  The physical re-sort interval and the number
+of steps per captured graph are tuned with the same timing loop.
+"""
+import time
+
+from . import configs
+from .system import System  # noqa: F401  (re-exported for callers that build samples)
+
+_CACHE = {}
+
+DPD_SHAPES = [(4, 4, 5), (4, 4, 4), (3, 4, 4), (4, 4, 6), (3, 4, 5)]
+MDPD_SHAPES = [(3, 3, 4), (3, 3, 3), (2, 3, 4), (3, 3, 5), (3, 4, 4)]
+
+
+def _time_force(ctx, cfg, brick, precision, steps):
+    import gpumdpd_amd as g
+    s = g.system_from_config(ctx, cfg, precision=precision, log_every=10 ** 6, upload=False)
+    try:
+        s.set_tuning(brick=brick)
+        s.upload(cfg["id"], cfg["type"], cfg["pos"], cfg["vel"], cfg["masses"])
+        got = s.tuning()
+        s.zero_step()
+        s.advance(2)
+        ctx.sync()
+        ctx.kernel_timing(True)
+        t0 = time.perf_counter()
+        s.advance(steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        ms, n, _ = ctx.kernel_timing_get()
+        ctx.kernel_timing(False)
+        s.finish()
+        return (got["bx"], got["by"], got["bz"]), ms / max(n, 1), wall / steps
+    finally:
+        s.destroy()
+
+
+def tune_brick(ctx, style, precision, n_sample=60000, steps=6, shapes=None, verbose=False):
+    """Fastest brick shape for replicas of `style` ("dpd" at rho=3, "mdpd" at
+    rho=6) in `precision`, from force-kernel time on an n_sample-bead replica."""
+    key = (style, precision)
+    if key in _CACHE:
+        return _CACHE[key]
+    cfg = configs.replica_system(style, n_sample, 12345)
+    shapes = shapes or (DPD_SHAPES if style == "dpd" else MDPD_SHAPES)
+    best = None
+    for b in shapes:
+        try:
+            got, kms, sms = _time_force(ctx, cfg, b, precision, steps)
+        except Exception as e:  # a shape that does not fit LDS is skipped
+            if verbose:
+                print(f"tune {style}: {b} skipped ({e})", flush=True)
+            continue
+        if verbose:
+            print(f"tune {style}: brick {got} force {kms:.3f} ms step {sms * 1e3:.3f} ms", flush=True)
+        if best is None or kms < best[1]:
+            best = (got, kms)
+    if best is None:
+        raise RuntimeError(f"no brick shape fits for {style}")
+    _CACHE[key] = best[0]
+    return best[0]

@@ -1,0 +1,133 @@
+"""Serving loop: admission by slots and HBM budget, continuous batching, and
+results that are the same alone and in the mix.
+
+The CPU tests drive the loop with a stand-in context that records calls (no
+GPU); the GPU test runs real replicas through the C ABI."""
+import numpy as np
+import pytest
+
+from gpumdpd_amd import serve
+
+
+class FakeSys:
+    def __init__(self, log, req):
+        self.log, self.req, self.step = log, req, 0
+        self.midstep = False
+
+    def zero_step(self):
+        self.midstep = True
+        self.log.append(("zero", self.req.k))
+
+    def finish(self):
+        self.log.append(("finish", self.req.k, self.step))
+
+    def destroy(self):
+        self.log.append(("destroy", self.req.k))
+
+    def download(self):
+        return {"pos": np.zeros((1, 3)), "vel": np.zeros((1, 3))}
+
+    def thermo_rows(self):
+        return np.zeros((0, 12))
+
+
+class FakeCtx:
+    def __init__(self):
+        self.log = []
+        self.max_batch = 0
+
+    def batch_advance(self, systems, k):
+        assert all(s.midstep for s in systems)
+        self.max_batch = max(self.max_batch, len(systems))
+        for s in systems:
+            s.step += k
+        self.log.append(("batch", tuple(s.req.k for s in systems), k))
+
+    def sync(self):
+        pass
+
+
+@pytest.fixture
+def fake(monkeypatch):
+    ctx = FakeCtx()
+    monkeypatch.setattr(serve, "_create", lambda c, r, *a: FakeSys(c.log, r))
+    return ctx
+
+
+def reqs_of(sizes, steps, arrivals=None, nbytes=None):
+    out = []
+    for k, n in enumerate(sizes):
+        r = serve.Request(k, "dpd", n, k + 1, arrivals[k] if arrivals else 0.0, steps[k])
+        r.bytes = nbytes[k] if nbytes else n * 300
+        out.append(r)
+    return out
+
+
+def test_every_request_runs_its_steps_exactly(fake):
+    reqs = reqs_of([100] * 7, [5, 12, 1, 33, 16, 17, 2])
+    m = serve.serve(fake, reqs, max_slots=3, chunk=4, realtime=False)
+    fin = {e[1]: e[2] for e in fake.log if e[0] == "finish"}
+    # zero_step is step 0 of steps; advance covers steps - 1 more
+    assert fin == {r.k: r.steps - 1 for r in reqs}
+    assert m["replica_steps"] == sum(r.steps for r in reqs)
+    assert fake.max_batch == 3
+
+
+def test_slots_and_budget_bound_admission(fake):
+    reqs = reqs_of([100] * 6, [9] * 6, nbytes=[40, 40, 40, 40, 40, 40])
+    serve.serve(fake, reqs, max_slots=5, hbm_budget=100, chunk=4, realtime=False)
+    assert fake.max_batch == 2  # the budget (100 B) holds two 40-B replicas
+    live, peak = set(), 0
+    for e in fake.log:
+        if e[0] == "zero":
+            live.add(e[1])
+        elif e[0] == "destroy":
+            live.discard(e[1])
+        peak = max(peak, len(live))
+    assert peak == 2
+    assert {e[1] for e in fake.log if e[0] == "destroy"} == set(range(6))
+
+
+def test_latency_samples_cover_every_step(fake):
+    reqs = reqs_of([100, 200], [10, 21])
+    m = serve.serve(fake, reqs, max_slots=2, chunk=8, realtime=False)
+    assert m["step_latency_p50_ms"] >= 0 and m["step_latency_p99_ms"] >= m["step_latency_p50_ms"]
+    assert m["requests"] == 2
+
+
+def test_request_stream_is_deterministic_and_in_range():
+    a = serve.make_requests(5, n=512)
+    b = serve.make_requests(5, n=512)
+    assert [(r.style, r.n, r.seed, r.arrival) for r in a] == [(r.style, r.n, r.seed, r.arrival) for r in b]
+    n = np.array([r.n for r in a])
+    assert n.min() >= 2000 and n.max() <= 200000
+    frac_dpd = np.mean([r.style == "dpd" for r in a])
+    assert 0.6 < frac_dpd < 0.8
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert 0.004 < gaps.mean() < 0.006  # Poisson at 200/s
+
+
+@pytest.mark.gpu
+def test_served_mix_equals_alone():
+    import gpumdpd_amd as g
+    ctx = g.Context(0)
+    try:
+        reqs = serve.make_requests(5, n=10, steps=41, n_max=12000, n_scale=0.2)
+        serve.prepare_inputs(reqs)
+        keep = [0, 3, 7, 9]
+        print("case: served mix of 10 replicas, 4 slots, 7-step chunks", flush=True)
+        m = serve.serve(ctx, reqs, max_slots=4, chunk=7, log_every=10, keep=keep, realtime=False)
+        assert m["replica_steps"] == sum(r.steps for r in reqs)
+        for k in keep:
+            r = reqs[k]
+            s = g.system_from_config(ctx, r.cfg, precision=g.PREC_F32, log_every=10)
+            s.zero_step()
+            s.advance(r.steps - 1)
+            s.finish()
+            d = s.download()
+            pos, vel, rows = m["kept"][k]
+            assert np.array_equal(d["pos"], pos) and np.array_equal(d["vel"], vel), f"request {k}"
+            assert np.array_equal(s.thermo_rows(), rows)
+            s.destroy()
+    finally:
+        ctx.close()
