@@ -121,12 +121,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device (reference kernel for scaling rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-chunk", type=int, default=16)
     ap.add_argument("--plain-every", type=int, default=8)
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(2 ** 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(this build runs a single system or a batch of replicas on one MI355X)", file=sys.stderr)

@@ -360,9 +360,12 @@ int step_once(gdpd_ctx *ctx, Tables &T, int mode)
     launch_integrate(ctx->d_reps, T.beads, T.n_beads, mode | kModeLogAuto, ctx->stream);
     launch_step_end(ctx->d_reps, T.rep_list, T.n_reps, 1, 1, 1, ctx->stream);
     if (mode & kModeDrift) {
+        // the order of beads inside a cell is whatever the rank atomics gave:
+        // every per-bead and per-brick sum is fixed point and each pair's
+        // contribution is the same from either side, so results do not depend
+        // on it
         launch_scan(ctx->d_reps, T.scan, T.n_scan, T.rep_chunk_off, T.rep_list, T.n_reps, T.chunk_sums, ctx->stream);
         launch_scatter(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
-        launch_cellsort(ctx->d_reps, T.cells, T.n_cells, ctx->stream);
     }
     return check_launch(ctx);
 }

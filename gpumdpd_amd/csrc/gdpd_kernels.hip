@@ -133,6 +133,11 @@ __device__ __forceinline__ void block_sum(double (&acc)[NV], double *red, double
 // batch).
 constexpr double kThermoFix = 4294967296.0;
 
+// LDS hash table of the per-block cell grouping in k_integrate
+constexpr int kRankBits = 9;
+constexpr int kRankSlots = 1 << kRankBits;
+static_assert(kRankSlots > kBlock, "the rank table must hold every key of a block");
+
 __device__ __forceinline__ long long wave_sum_i64(long long v)
 {
 #pragma unroll
@@ -142,6 +147,28 @@ __device__ __forceinline__ long long wave_sum_i64(long long v)
         v += ((long long)hi << 32) | (unsigned)lo;
     }
     return v;
+}
+
+__device__ __forceinline__ long long thermo_fix(double v) { return __double2ll_rn(v * kThermoFix); }
+
+// exact block sum of per-thread integer partials; thread k < NV of the block
+// gets the total of component k in out[k]
+template <int NV>
+__device__ __forceinline__ void block_sum_i64(const long long (&acc)[NV], long long *red, long long *out)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const long long s = wave_sum_i64(acc[k]);
+        if (lane == 0) red[w * NV + k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        long long s = 0;
+        for (int ww = 0; ww < kBlock / 64; ++ww) s += red[ww * NV + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
 }
 
 template <int NV>
@@ -257,9 +284,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RepDev *__restrict__ r
     else atomicOr(&R.state->flags, kFlagStageOverflow);
 }
 
-// sort each cell's beads by id so the in-cell order, and with it every
-// per-bead summation order, depends only on the state: not on atomic
-// scheduling, on the slot layout (re-sort interval) or on the batch
+// sort each cell's beads by id.  Used by the initial binning only, so a freshly
+// uploaded system starts from one fixed slot layout; the step path needs no
+// in-cell order (its sums are fixed point and pair contributions symmetric)
 __global__ __launch_bounds__(kBlock) void k_cellsort(const RepDev *__restrict__ reps,
                                                      const WorkItem *__restrict__ items, int nitems)
 {
@@ -685,7 +712,7 @@ __device__ __forceinline__ int lanes_below(unsigned long long m)
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real> &L, const unsigned *q, int qn,
                                            int nt, uint32_t seed_step, bool log, bool perbead,
-                                           double (&acc)[kThermoParts])
+                                           long long (&acc)[kThermoParts])
 {
     using real4 = typename Vec4T<real>::type;
     const int lane = threadIdx.x & 63;
@@ -756,10 +783,13 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             }
             const real w6[6] = {(real)0.5 * dx * ffx, (real)0.5 * dx * ffy, (real)0.5 * dx * ffz,
                                 (real)0.5 * dy * ffy, (real)0.5 * dy * ffz, (real)0.5 * dz * ffz};
-            const double sides = jown ? 2.0 : 1.0;
-            acc[1] += (double)pe_i + (jown ? (double)pe_j : 0.0);
+            // per-pair fixed point: the brick sums do not depend on pair order
+            acc[1] += thermo_fix((double)pe_i) + (jown ? thermo_fix((double)pe_j) : 0ll);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc[3 + c] += sides * (double)w6[c];
+            for (int c = 0; c < 6; ++c) {
+                const long long wq = thermo_fix((double)w6[c]);
+                acc[3 + c] += jown ? 2 * wq : wq;
+            }
             if (perbead) {
                 // hook-path log step: per-bead fixed-point sums in the pe/virial arrays
                 const int N = R.N, si = L.oslot[oi], sj = jown ? L.oslot[oj] : 0;
@@ -786,7 +816,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
                                             const BrickParams &bp, int O, int nt, real cut2, uint32_t seed_step,
-                                            bool log, double (&acc)[kThermoParts])
+                                            bool log, long long (&acc)[kThermoParts])
 {
     using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -913,9 +943,9 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
         if (threadIdx.x == 0) atomicOr(&R.state->flags, kFlagStageOverflow);
         return;
     }
-    double acc[kThermoParts];
+    long long acc[kThermoParts];
 #pragma unroll
-    for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
+    for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
     brick_pairs<real, 2, true>(R, g, L, bp, O, R.ntypes, (real)R.rdmax2, 0u, false, acc);
     for (int o = threadIdx.x; o < O; o += kBlock)
         R.rho[L.oslot[o]] = (double)L.facc[o] * FixScale<real>::inv;
@@ -948,9 +978,9 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     const int t = threadIdx.x;
     const int N = R.N;
     const uint32_t seed_step = R.seed + (uint32_t)step;
-    double acc[kThermoParts];
+    long long acc[kThermoParts];
 #pragma unroll
-    for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
+    for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
     brick_pairs<real, STYLE, false>(R, g, L, bp, O, R.ntypes, (real)R.rcmax2, seed_step, log, acc);
     for (int o = t; o < O; o += kBlock) {
         const int slot = L.oslot[o];
@@ -961,7 +991,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     }
     if (log && !bp.perbead) {
         long long *red = (long long *)L.red, *out = red + 48;
-        block_sum_fix<kThermoParts>(acc, red, out);
+        block_sum_i64<kThermoParts>(acc, red, out);
         if (t < kThermoParts) ((long long *)R.thermo_part)[(size_t)it.local * kThermoParts + t] = out[t];
     }
 }
@@ -979,6 +1009,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
                                                       const WorkItem *__restrict__ items, int nitems, int mode)
 {
     __shared__ long long red[64];
+    __shared__ int hkey[kRankSlots], hcnt[kRankSlots], hbase[kRankSlots];
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int cur = R.state->cur;
@@ -990,6 +1021,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0.0;
     bool bad = false;
+    int my_key = -1;
     if (s < N) {
         const double *X = R.x[cur], *V = R.v[cur];
         double fx = R.force[s], fy = R.force[N + s], fz = R.force[2 * N + s];
@@ -1075,7 +1107,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             Vo[N + s] = vyg;
             Vo[2 * N + s] = vzg;
             R.key[s] = key;
-            R.rank[s] = atomicAdd(&R.cell_count[key], 1);
+            my_key = key;
         } else {
             double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
             Xo[s] = xg;
@@ -1087,6 +1119,35 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
         }
     }
     if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
+    if (mode & kModeDrift) {
+        // rank of each bead in its new cell.  Neighbouring slots share cells,
+        // so the block first groups its beads by cell in an LDS hash table
+        // (local ranks from LDS atomics), then takes one global atomic per
+        // distinct cell for the group's base.  Ranks within a cell need no
+        // particular order.
+        int h = 0, lrank = 0;
+        for (int e = threadIdx.x; e < kRankSlots; e += kBlock) {
+            hkey[e] = -1;
+            hcnt[e] = 0;
+        }
+        __syncthreads();
+        if (my_key >= 0) {
+            h = (int)(((unsigned)my_key * 2654435761u) >> (32 - kRankBits));
+            // at most kBlock distinct keys in kRankSlots > kBlock slots: every
+            // probe sequence reaches the key or an empty slot
+            for (;;) {
+                const int prev = atomicCAS(&hkey[h], -1, my_key);
+                if (prev == -1 || prev == my_key) break;
+                h = (h + 1) & (kRankSlots - 1);
+            }
+            lrank = atomicAdd(&hcnt[h], 1);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < kRankSlots; e += kBlock)
+            if (hkey[e] >= 0) hbase[e] = atomicAdd(&R.cell_count[hkey[e]], hcnt[e]);
+        __syncthreads();
+        if (my_key >= 0) R.rank[s] = hbase[h] + lrank;
+    }
     if (log) {
         long long *out = red + 48;
         block_sum_fix<kThermoParts>(acc, red, out);

@@ -1,7 +1,9 @@
 """The fused, hook, batched and C++-shell paths compute the same step.
 
-Per-bead sums run in a canonical order (cells visited in a fixed order, beads
-within a cell by id), so every path below must agree bit for bit.
+Per-bead and per-brick sums are 64-bit fixed point (exact, order-independent)
+and a pair's contribution is the same whichever of its beads finds it, so the
+order of beads in cells, slots and launches does not matter: every path below
+must agree bit for bit.
 """
 import os
 import subprocess
