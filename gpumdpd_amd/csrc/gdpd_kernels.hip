@@ -888,8 +888,13 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
             const bool test_order = thin && s < 4;
             for (int k = 0; k < maxlen; k += 2) {
                 if (qn > qw - 128) {
-                    eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
-                    qn = 0;
+                    // evaluate whole 64-pair rounds only; the remainder (< 64
+                    // entries) moves to the front of the queue
+                    const int ne = qn & ~63;
+                    eval_queue<real, STYLE, DENSITY>(R, L, q, ne, nt, seed_step, log, bp.perbead, acc);
+                    const int rem = qn - ne;
+                    if (lane < rem) q[lane] = q[ne + lane];
+                    qn = rem;
                 }
                 // the pair of loads starts at most at smax - 1 (pos holds smax + 1
                 // entries); lanes past their segment are dropped by the k < len test
