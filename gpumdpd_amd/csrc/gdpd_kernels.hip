@@ -696,12 +696,9 @@ __device__ __forceinline__ void lds_sub(long long *p, long long v)
                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ int wave_max(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+// wavefront maximum (DPP row/wave shifts, result read from one lane)
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 
 __device__ __forceinline__ int lanes_below(unsigned long long m)
 {
