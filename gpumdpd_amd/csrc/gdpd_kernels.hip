@@ -71,16 +71,10 @@ __device__ __forceinline__ double wrap_axis(double x, double lo, double hi, doub
     return x;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int x)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
+// wavefront inclusive prefix sum and 64-bit sum (DPP, no LDS round trips)
+extern "C" __device__ int __ockl_wfscan_add_i32(int, bool);
+extern "C" __device__ long __ockl_wfred_add_i64(long);
+__device__ __forceinline__ int wave_incl_scan(int x) { return __ockl_wfscan_add_i32(x, true); }
 
 // exclusive scan over the 256 threads; every thread must call it
 __device__ __forceinline__ int block_excl_scan(int v, int *wbuf, int &total)
@@ -138,16 +132,7 @@ constexpr int kRankBits = 9;
 constexpr int kRankSlots = 1 << kRankBits;
 static_assert(kRankSlots > kBlock, "the rank table must hold every key of a block");
 
-__device__ __forceinline__ long long wave_sum_i64(long long v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const int lo = __shfl_xor((int)(unsigned)(v & 0xFFFFFFFFll), o, 64);
-        const int hi = __shfl_xor((int)(v >> 32), o, 64);
-        v += ((long long)hi << 32) | (unsigned)lo;
-    }
-    return v;
-}
+__device__ __forceinline__ long long wave_sum_i64(long long v) { return (long long)__ockl_wfred_add_i64((long)v); }
 
 __device__ __forceinline__ long long thermo_fix(double v) { return __double2ll_rn(v * kThermoFix); }
 
