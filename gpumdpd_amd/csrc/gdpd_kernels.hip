@@ -127,6 +127,9 @@ __device__ __forceinline__ void block_sum(double (&acc)[NV], double *red, double
 // batch).
 constexpr double kThermoFix = 4294967296.0;
 
+// face classes of a brick's owned cells (see face_class)
+constexpr int kSigClasses = 32;
+
 // LDS hash table of the per-block cell grouping in k_integrate
 constexpr int kRankBits = 9;
 constexpr int kRankSlots = 1 << kRankBits;
@@ -366,7 +369,7 @@ template <typename real> struct BrickLds {
     real4 *pos, *vel;          // xyz + type, vxyz + id bits
     real *rh;                  // staged density (mDPD force pass)
     unsigned short *own;       // owned index, or 0xFFFF for halo beads
-    int *c_start, *cinfo, *gstart, *colpre, *wbuf, *oslot, *oinfo;
+    int *c_start, *cinfo, *gstart, *cellpre, *sigcnt, *wbuf, *oslot, *oinfo;
     long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
 };
@@ -376,7 +379,7 @@ __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t
 // LDS layout as byte offsets; pointers are rebuilt inline from the __shared__
 // base so they stay in the LDS address space.
 struct LdsOffsets {
-    unsigned coef, red, pos, vel, rh, own, c_start, cinfo, gstart, colpre, wbuf, oslot, oinfo, facc, queue, total;
+    unsigned coef, red, pos, vel, rh, own, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oslot, oinfo, facc, queue, total;
 };
 
 template <typename real>
@@ -396,7 +399,8 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
     GDPD_TAKE(cinfo, sizeof(int) * p.rmax)
     GDPD_TAKE(gstart, sizeof(int) * p.rmax)
-    GDPD_TAKE(colpre, sizeof(int) * (p.ncol + 1))
+    GDPD_TAKE(cellpre, sizeof(int) * p.rmax)
+    GDPD_TAKE(sigcnt, sizeof(int) * (kSigClasses + 1))
     GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
     GDPD_TAKE(oslot, sizeof(int) * p.omax)
     GDPD_TAKE(oinfo, sizeof(int) * p.omax)
@@ -421,7 +425,8 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.c_start = (int *)(smem + o.c_start);
     L.cinfo = (int *)(smem + o.cinfo);
     L.gstart = (int *)(smem + o.gstart);
-    L.colpre = (int *)(smem + o.colpre);
+    L.cellpre = (int *)(smem + o.cellpre);
+    L.sigcnt = (int *)(smem + o.sigcnt);
     L.wbuf = (int *)(smem + o.wbuf);
     L.oslot = (int *)(smem + o.oslot);
     L.oinfo = (int *)(smem + o.oinfo);
@@ -475,31 +480,15 @@ __device__ __forceinline__ double id_to_real(int id, double) { return __longlong
 __device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_longlong(v); }
 
-// Owned beads are grouped by brick column (rx, ry).  Interior columns
-// (rx >= 2 and ry >= 2) come first: their beads find no partners in the lower
-// neighbour runs except in z-halo cells, so waves of interior beads skip most of
-// those runs together.
-__device__ __forceinline__ void column_of(const BrickGeom &g, int q, int &rx, int &ry)
-{
-    const int ni = (g.ext[0] - 1) * (g.ext[1] - 1);
-    if (q < ni) {
-        rx = 2 + q / (g.ext[1] - 1);
-        ry = 2 + q % (g.ext[1] - 1);
-    } else if (q - ni < g.ext[1]) {
-        rx = 1;
-        ry = 1 + (q - ni);
-    } else {
-        rx = 2 + (q - ni - g.ext[1]);
-        ry = 1;
-    }
-}
+// Owned beads are numbered cell by cell, grouped by the cell's face class: the
+// brick faces the cell touches (x-low, y-low, y-high, z-low, z-high).  Cells of
+// one class have the same set of non-empty backward (halo-only) candidate
+// segments, so a wavefront of beads from one class skips the same segments
+// together; interior cells (class 0) come first.
 
-__device__ __forceinline__ int column_index(const BrickGeom &g, int rx, int ry)
+__device__ __forceinline__ int face_class(const BrickGeom &g, int rx, int ry, int rz)
 {
-    const int ni = (g.ext[0] - 1) * (g.ext[1] - 1);
-    if (rx >= 2 && ry >= 2) return (rx - 2) * (g.ext[1] - 1) + (ry - 2);
-    if (rx == 1) return ni + (ry - 1);
-    return ni + g.ext[1] + (rx - 2);
+    return ((rx == 1) << 4) | ((ry == 1) << 3) | ((ry == g.ext[1]) << 2) | ((rz == 1) << 1) | (rz == g.ext[2]);
 }
 
 // Stage the brick + halo.  Returns (uniformly) the number of owned beads, or
@@ -539,22 +528,33 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             if (rc < g.nreg) L.c_start[rc] = run;
             run += v[k];
         }
-        if (t == 0) {
-            L.c_start[g.nreg] = total;
-            // owned beads: brick columns (rx, ry) = 1..ext, cells rz = 1..ext[2]
-            L.colpre[0] = 0;
-            const int ncol = g.ext[0] * g.ext[1];
-            for (int q = 0; q < ncol; ++q) {
-                int rx, ry;
-                column_of(g, q, rx, ry);
-                const int c = (rx * g.RY + ry) * g.RZ;
-                L.colpre[q + 1] = L.colpre[q] + (L.c_start[c + g.ext[2] + 1] - L.c_start[c + 1]);
-            }
-        }
+        if (t == 0) L.c_start[g.nreg] = total;
+        if (t <= kSigClasses) L.sigcnt[t] = 0;
+    }
+    __syncthreads();
+    // owned-cell offsets: bead counts per face class (LDS atomics; the order
+    // of cells inside a class is immaterial), then a scan over the classes
+    const int nown = g.ext[0] * g.ext[1] * g.ext[2];
+    for (int oc = t; oc < nown; oc += kBlock) {
+        const int rz = 1 + oc % g.ext[2], ry = 1 + (oc / g.ext[2]) % g.ext[1], rx = 1 + oc / (g.ext[2] * g.ext[1]);
+        const int rc = (rx * g.RY + ry) * g.RZ + rz;
+        L.cellpre[oc] = atomicAdd(&L.sigcnt[face_class(g, rx, ry, rz)], L.c_start[rc + 1] - L.c_start[rc]);
+    }
+    __syncthreads();
+    if (t < 64) {
+        const int v = t < kSigClasses ? L.sigcnt[t] : 0;
+        const int inc = wave_incl_scan(v);
+        if (t < kSigClasses) L.sigcnt[t] = inc - v;
+        if (t == kSigClasses - 1) L.sigcnt[kSigClasses] = inc;
+    }
+    __syncthreads();
+    for (int oc = t; oc < nown; oc += kBlock) {
+        const int rz = 1 + oc % g.ext[2], ry = 1 + (oc / g.ext[2]) % g.ext[1], rx = 1 + oc / (g.ext[2] * g.ext[1]);
+        L.cellpre[oc] += L.sigcnt[face_class(g, rx, ry, rz)];
     }
     __syncthreads();
     const int S = L.c_start[g.nreg];
-    const int O = L.colpre[g.ext[0] * g.ext[1]];
+    const int O = L.sigcnt[kSigClasses];
     if (S > bp.smax || O > bp.omax) return -1;
 
     const int N = R.N;
@@ -625,8 +625,8 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             if (need_rho) L.rh[k] = (real)R.rho[sl[u]];
             unsigned short ow = 0xFFFF;
             if (inner) {
-                const int qc = column_index(g, rx, ry);
-                const int o = L.colpre[qc] + (k - L.c_start[(rx * g.RY + ry) * g.RZ + 1]);
+                const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
+                const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
                 ow = (unsigned short)o;
                 L.oslot[o] = sl[u];
                 L.oinfo[o] = k | (rc2 << 16);
