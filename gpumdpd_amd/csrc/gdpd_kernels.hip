@@ -127,6 +127,9 @@ __device__ __forceinline__ void block_sum(double (&acc)[NV], double *red, double
 // batch).
 constexpr double kThermoFix = 4294967296.0;
 
+// owned-index field of a staged bead's tag for a halo bead
+constexpr int kHaloOwner = 0xFFFF;
+
 // face classes of a brick's owned cells (see face_class)
 constexpr int kSigClasses = 32;
 
@@ -366,9 +369,8 @@ template <typename real> struct BrickLds {
     using real4 = typename Vec4T<real>::type;
     PairCoef<real> *coef;
     double *red;
-    real4 *pos, *vel;          // xyz + type, vxyz + id bits
+    real4 *pos, *vel;          // xyz + tag bits (type | owned index << 8), vxyz + id bits
     real *rh;                  // staged density (mDPD force pass)
-    unsigned short *own;       // owned index, or 0xFFFF for halo beads
     int *c_start, *cinfo, *gstart, *cellpre, *sigcnt, *wbuf, *oslot, *oinfo;
     long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
@@ -379,7 +381,7 @@ __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t
 // LDS layout as byte offsets; pointers are rebuilt inline from the __shared__
 // base so they stay in the LDS address space.
 struct LdsOffsets {
-    unsigned coef, red, pos, vel, rh, own, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oslot, oinfo, facc, queue, total;
+    unsigned coef, red, pos, vel, rh, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oslot, oinfo, facc, queue, total;
 };
 
 template <typename real>
@@ -395,7 +397,6 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     GDPD_TAKE(pos, sizeof(real) * 4 * (p.smax + 1))
     GDPD_TAKE(vel, sizeof(real) * 4 * p.smax)
     GDPD_TAKE(rh, sizeof(real) * (p.style == 2 ? p.smax : 1))
-    GDPD_TAKE(own, sizeof(unsigned short) * p.smax)
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
     GDPD_TAKE(cinfo, sizeof(int) * p.rmax)
     GDPD_TAKE(gstart, sizeof(int) * p.rmax)
@@ -421,7 +422,6 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.pos = (real4 *)(smem + o.pos);
     L.vel = (real4 *)(smem + o.vel);
     L.rh = (real *)(smem + o.rh);
-    L.own = (unsigned short *)(smem + o.own);
     L.c_start = (int *)(smem + o.c_start);
     L.cinfo = (int *)(smem + o.cinfo);
     L.gstart = (int *)(smem + o.gstart);
@@ -474,7 +474,6 @@ __device__ __forceinline__ int wrap_code(int c, int n, int &code)
     return c;
 }
 
-template <typename real> __device__ __forceinline__ real type_to_real(int t) { return (real)t; }
 __device__ __forceinline__ float id_to_real(int id, float) { return __int_as_float(id); }
 __device__ __forceinline__ double id_to_real(int id, double) { return __longlong_as_double((long long)id); }
 __device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
@@ -607,12 +606,20 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             const double shy = (double)(((ci >> 26) & 3) - 1) * R.L[1];
             const double shz = (double)(((ci >> 28) & 3) - 1) * R.L[2];
             const bool inner = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
+            int ow = kHaloOwner;
+            if (inner) {
+                const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
+                const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
+                ow = o;
+                L.oslot[o] = sl[u];
+                L.oinfo[o] = k | (rc2 << 16);
+            }
             real4 p;
             p.x = (real)((xs[u] + shx) - g.org[0]);
             p.y = (real)((ys[u] + shy) - g.org[1]);
             p.z = (real)((zs[u] + shz) - g.org[2]);
-            // type in w, negative for halo beads
-            p.w = type_to_real<real>(inner ? tys[u] : -tys[u]);
+            // w carries the tag: type | owned index << 8 (kHaloOwner for halo)
+            p.w = id_to_real(tys[u] | (ow << 8), (real)0);
             L.pos[k] = p;
             if (need_vel) {
                 real4 q;
@@ -623,15 +630,6 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
                 L.vel[k] = q;
             }
             if (need_rho) L.rh[k] = (real)R.rho[sl[u]];
-            unsigned short ow = 0xFFFF;
-            if (inner) {
-                const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
-                const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
-                ow = (unsigned short)o;
-                L.oslot[o] = sl[u];
-                L.oinfo[o] = k | (rc2 << 16);
-            }
-            L.own[k] = ow;
         }
     }
     for (int k = t; k < R.ntypes * R.ntypes; k += kBlock) L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
@@ -702,11 +700,12 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         const unsigned ent = q[e];
         const int li = ent & 0xFFFF, lj = ent >> 16;
         const real4 pi = L.pos[li], pj = L.pos[lj];
-        const int oi = L.own[li], oj = L.own[lj];
-        const bool jown = oj != 0xFFFF;
+        const int tgi = real_to_id(pi.w), tgj = real_to_id(pj.w);
+        const int oi = tgi >> 8, oj = tgj >> 8;
+        const bool jown = oj != kHaloOwner;
         const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const real r2 = dx * dx + dy * dy + dz * dz;
-        const PairCoef<real> &C = L.coef[((int)fabs(pi.w) - 1) * nt + ((int)fabs(pj.w) - 1)];
+        const PairCoef<real> &C = L.coef[((tgi & 255) - 1) * nt + ((tgj & 255) - 1)];
         if (DENSITY) {
             const real r = sqrt(r2);
             // force/pair_mdpd.cu:266-289
@@ -891,8 +890,8 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                 bool pa = k < len && (DENSITY ? (r20 < cut2) : (r20 <= cut2));
                 bool pb = k + 1 < len && (DENSITY ? (r21 < cut2) : (r21 <= cut2));
                 if (test_order) {
-                    pa = pa && (la > li || pj0.w < (real)0);
-                    pb = pb && (la + 1 > li || pj1.w < (real)0);
+                    pa = pa && (la > li || (real_to_id(pj0.w) >> 8) == kHaloOwner);
+                    pb = pb && (la + 1 > li || (real_to_id(pj1.w) >> 8) == kHaloOwner);
                 }
                 const unsigned long long ma = __ballot(pa);
                 const unsigned long long mb = __ballot(pb);
