@@ -27,21 +27,26 @@ def hipcc_command(out=LIB):
     ] + srcs
 
 
-DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "plugin_driver.cpp")
-DRIVER_BIN = os.path.join(ROOT, "tests", "cpp", "plugin_driver")
+CPP_DIR = os.path.join(ROOT, "tests", "cpp")
+# C++ programs over the plugin shells used by the tests (link libgdpd.so)
+DRIVERS = ["plugin_driver", "data_roundtrip", "deck_runner"]
+DRIVER_SRC = os.path.join(CPP_DIR, "plugin_driver.cpp")
 
 
 def build_driver(verbose=False):
-    """The C++ plugin-shell driver used by the GPU tests (links libgdpd.so)."""
-    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", DRIVER_BIN,
-           "-L", HERE, "-lgdpd", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../gpumdpd_amd"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("g++ failed building the plugin driver")
-    return DRIVER_BIN
+    for name in DRIVERS:
+        src = os.path.join(CPP_DIR, name + ".cpp")
+        if not os.path.exists(src):
+            continue
+        cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), src, "-o",
+               os.path.join(CPP_DIR, name), "-L", HERE, "-lgdpd", f"-Wl,-rpath,{HERE}",
+               "-Wl,-rpath,$ORIGIN/../../gpumdpd_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed building {name}")
 
 
 def build(verbose=False):

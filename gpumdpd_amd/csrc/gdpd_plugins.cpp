@@ -3,14 +3,17 @@
 // Each class keeps the reference's method names and argument meaning; the
 // device work is one gdpd_* call per hook.  Deck parsing follows the
 // reference's commands (main/read_in.cu:40-290, force/pair_dpd.cu:19-173,
-// force/pair_mdpd.cu:21-192, force/bond_harmonic.cu:19-147, fix/fix.cu:60-115).
+// force/pair_mdpd.cu:21-192, force/bond_harmonic.cu:19-147, fix/fix.cu:60-115);
+// data files follow system/read_data.cu.
 #include "../../include/gdpd_plugins.hpp"
 
 #include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <fstream>
 #include <iostream>
+#include <map>
 #include <sstream>
 
 namespace gdpd_host {
@@ -112,9 +115,275 @@ void read_input(System &s, const std::string &text)
         } else if (key == "run") {
             if (a.size() != 1) illegal(l, "illegal run command", "run N");
             s.run_p.nsteps = (unsigned)to_i(a[0], l, "N", "run N");
+        } else if (key == "read_data") {
+            if (a.size() != 1) illegal(l, "illegal read_data command", "read_data filename");
+            s.run_p.input_data_name = a[0];
+        } else if (key == "write_data") {
+            if (a.size() != 1) illegal(l, "illegal write_data command", "write_data filename");
+            s.run_p.out_data_name = a[0];
+            s.run_p.write_data_flag = true;
+        } else if (key == "dump") {
+            // trajectory output is not part of this library
+            std::fprintf(stderr, "gdpd: ignoring dump command (no trajectory output): %s\n", l.c_str());
+        } else if (key == "angle_style" || key == "angle_coeff" || key == "compute") {
+            throw Error("\n   Error: " + key + " is not supported by this build\n   Line: " + l);
+        } else if (key != "pair_style" && key != "pair_coeff" && key != "bond_style" && key != "bond_coeff" &&
+                   key != "group" && key != "fix") {
+            // main/read_in.cu:295-305
+            throw Error("unknown command " + key);
         }
     });
     if (s.run_p.log_f == 0) throw Error("thermo frequency must be positive");
+}
+
+// ---- data files (system/read_data.cu) ------------------------------------------
+
+namespace {
+
+std::string strip_comment(const std::string &line)
+{
+    std::string t = line.substr(0, line.find('#'));
+    const size_t a = t.find_first_not_of(" \t\r\n");
+    if (a == std::string::npos) return "";
+    const size_t b = t.find_last_not_of(" \t\r\n");
+    return t.substr(a, b - a + 1);
+}
+
+bool ends_with(const std::string &t, const std::string &suffix)
+{
+    return t.size() >= suffix.size() && t.compare(t.size() - suffix.size(), suffix.size(), suffix) == 0;
+}
+
+const char *kSections[] = {"Masses", "Atoms", "Velocities", "Bonds", "Angles", "Pair Coeffs", "Bond Coeffs",
+                           "Angle Coeffs"};
+
+bool is_section(const std::string &t)
+{
+    for (const char *k : kSections)
+        if (t == k) return true;
+    return false;
+}
+
+// next `count` non-blank rows after line i (comments stripped)
+std::vector<std::string> section_rows(const std::vector<std::string> &lines, size_t &i, long count,
+                                      const std::string &name)
+{
+    std::vector<std::string> rows;
+    while (i < lines.size() && (long)rows.size() < count) {
+        const std::string t = strip_comment(lines[i++]);
+        if (t.empty()) continue;
+        if (is_section(t)) break;
+        rows.push_back(t);
+    }
+    if ((long)rows.size() != count)
+        throw Error("data file: section " + name + " has " + std::to_string(rows.size()) + " rows, expected " +
+                    std::to_string(count));
+    return rows;
+}
+
+} // namespace
+
+void read_data(System &s, const std::string &path)
+{
+    std::ifstream f(path);
+    if (!f) throw Error("Cannot open data file: " + path);
+    std::vector<std::string> lines;
+    for (std::string l; std::getline(f, l);) lines.push_back(l);
+    // line 0 is the title; header counts and box lines follow until a section
+    long n_atoms = -1, n_bonds = 0, n_at = -1, n_bt = 0;
+    bool box[3] = {false, false, false};
+    size_t i = 1;
+    for (; i < lines.size(); ++i) {
+        const std::string t = strip_comment(lines[i]);
+        if (t.empty()) continue;
+        if (is_section(t)) break;
+        std::istringstream is(t);
+        if (ends_with(t, "atom types")) {
+            if (!(is >> n_at)) throw Error("Error reading number of atom types from line: " + lines[i]);
+        } else if (ends_with(t, "bond types")) {
+            if (!(is >> n_bt)) throw Error("Error reading number of bond types from line: " + lines[i]);
+        } else if (ends_with(t, "angle types") || ends_with(t, "angles")) {
+            long n = 0;
+            is >> n;
+            if (n > 0) throw Error("data file has angles: atom_style angle is not supported by this build");
+        } else if (ends_with(t, "atoms")) {
+            if (!(is >> n_atoms)) throw Error("Error reading number of atoms from line: " + lines[i]);
+        } else if (ends_with(t, "bonds")) {
+            if (!(is >> n_bonds)) throw Error("Error reading number of bonds from line: " + lines[i]);
+        } else if (ends_with(t, "xlo xhi")) {
+            if (!(is >> s.box.xlo >> s.box.xhi)) throw Error("Error reading xlo xhi values.");
+            box[0] = true;
+        } else if (ends_with(t, "ylo yhi")) {
+            if (!(is >> s.box.ylo >> s.box.yhi)) throw Error("Error reading ylo yhi values.");
+            box[1] = true;
+        } else if (ends_with(t, "zlo zhi")) {
+            if (!(is >> s.box.zlo >> s.box.zhi)) throw Error("Error reading zlo zhi values.");
+            box[2] = true;
+        } else {
+            throw Error("data file: unknown header line: " + lines[i]);
+        }
+    }
+    if (n_atoms <= 0) throw Error("data file: number of atoms is missing");
+    if (n_at <= 0) throw Error("data file: number of atom types is missing");
+    if (!box[0] || !box[1] || !box[2]) throw Error("data file: box bounds are missing");
+    if (s.box.xlo >= s.box.xhi || s.box.ylo >= s.box.yhi || s.box.zlo >= s.box.zhi)
+        throw Error("Invalid box dimensions.");
+    if (s.atom_style == 0 && n_bonds > 0) throw Error("data file has bonds but atom_style is atomic");
+    s.n_atoms = (int)n_atoms;
+    s.n_atomtypes = (int)n_at;
+    s.n_bonds = s.atom_style == 1 ? (int)n_bonds : 0;
+    s.n_bondtypes = s.atom_style == 1 ? (int)n_bt : 0;
+    s.box.lx = s.box.xhi - s.box.xlo;
+    s.box.ly = s.box.yhi - s.box.ylo;
+    s.box.lz = s.box.zhi - s.box.zlo;
+    s.box.hlx = 0.5 * s.box.lx;
+    s.box.hly = 0.5 * s.box.ly;
+    s.box.hlz = 0.5 * s.box.lz;
+    const size_t N = (size_t)n_atoms;
+    s.masses.assign((size_t)n_at, -1.0);
+    s.h_id.assign(N, 0);
+    s.h_type.assign(N, 0);
+    s.h_pos.assign(3 * N, 0.0);
+    s.h_vel.assign(3 * N, 0.0);
+    s.h_mol_id.assign(s.atom_style == 1 ? N : 0, 0);
+    s.h_bondlist.assign(3 * (size_t)s.n_bonds, -1);
+    s.has_velocities = false;
+    bool got_atoms = false, got_masses = false, got_bonds = false;
+    const double lo[3] = {s.box.xlo, s.box.ylo, s.box.zlo}, hi[3] = {s.box.xhi, s.box.yhi, s.box.zhi};
+    while (i < lines.size()) {
+        const std::string raw = lines[i];
+        const std::string t = strip_comment(raw);
+        ++i;
+        if (t.empty()) continue;
+        if (!is_section(t)) throw Error("data file: unexpected line: " + raw);
+        if (t == "Masses") {
+            for (const std::string &r : section_rows(lines, i, n_at, t)) {
+                std::istringstream is(r);
+                long ty;
+                double m;
+                if (!(is >> ty >> m) || ty < 1 || ty > n_at) throw Error("Error reading mass for atom type: " + r);
+                s.masses[(size_t)ty - 1] = m;
+            }
+            got_masses = true;
+        } else if (t == "Atoms") {
+            // the style hint after '#' (system/read_data.cu:27-58)
+            const size_t c = raw.find('#');
+            if (c != std::string::npos) {
+                std::istringstream hs(raw.substr(c + 1));
+                std::string hint;
+                hs >> hint;
+                const int style = hint == "atomic" ? 0 : hint == "bond" ? 1 : hint == "angle" ? 2 : -1;
+                if (style < 0) throw Error("Unknown atom_style in data file.");
+                if (style != s.atom_style)
+                    throw Error("atom_style in run.in is inconsistent with the style in data file.");
+            }
+            for (const std::string &r : section_rows(lines, i, n_atoms, t)) {
+                std::istringstream is(r);
+                long id, mol = 0, ty;
+                double x[3];
+                bool ok = s.atom_style == 1 ? (bool)(is >> id >> mol >> ty >> x[0] >> x[1] >> x[2])
+                                            : (bool)(is >> id >> ty >> x[0] >> x[1] >> x[2]);
+                if (!ok) throw Error("data file: bad Atoms row: " + r);
+                if (id == 0) throw Error("Atom ID cannot be 0.");
+                if (id < 1 || id > n_atoms) throw Error("data file: atom id out of range: " + r);
+                if (s.h_id[(size_t)id - 1] != 0) throw Error("data file: duplicate atom id: " + r);
+                if (ty < 1 || ty > n_at) throw Error("data file: atom type out of range: " + r);
+                const size_t k = (size_t)id - 1;
+                s.h_id[k] = (int)id;
+                s.h_type[k] = (int)ty;
+                if (s.atom_style == 1) s.h_mol_id[k] = (int)mol;
+                for (int d = 0; d < 3; ++d) {
+                    // Read_data::adjust_coordinate (system/read_data.cu:191-197)
+                    double v = x[d];
+                    while (v >= hi[d] || v < lo[d]) v -= (hi[d] - lo[d]) * ((v >= hi[d]) - (v < lo[d]));
+                    s.h_pos[3 * k + d] = v;
+                }
+            }
+            got_atoms = true;
+        } else if (t == "Velocities") {
+            for (const std::string &r : section_rows(lines, i, n_atoms, t)) {
+                std::istringstream is(r);
+                long id;
+                double v[3];
+                if (!(is >> id >> v[0] >> v[1] >> v[2]) || id < 1 || id > n_atoms)
+                    throw Error("data file: bad Velocities row: " + r);
+                for (int d = 0; d < 3; ++d) s.h_vel[3 * ((size_t)id - 1) + d] = v[d];
+            }
+            s.has_velocities = true;
+        } else if (t == "Bonds") {
+            if (s.atom_style != 1) throw Error("data file has a Bonds section but atom_style is atomic");
+            for (const std::string &r : section_rows(lines, i, n_bonds, t)) {
+                std::istringstream is(r);
+                long bid, bt, a, b;
+                if (!(is >> bid >> bt >> a >> b)) throw Error("data file: bad Bonds row: " + r);
+                if (bid == 0) throw Error("Bond ID cannot be 0.");
+                if (bid < 1 || bid > n_bonds || bt < 1 || bt > std::max(n_bt, 1L) || a < 1 || a > n_atoms ||
+                    b < 1 || b > n_atoms)
+                    throw Error("data file: bond entry out of range: " + r);
+                const size_t k = 3 * ((size_t)bid - 1);
+                s.h_bondlist[k] = (int)bt;
+                s.h_bondlist[k + 1] = (int)a;
+                s.h_bondlist[k + 2] = (int)b;
+            }
+            got_bonds = true;
+        } else if (t == "Angles") {
+            throw Error("data file has angles: atom_style angle is not supported by this build");
+        } else {
+            // coefficient sections: the deck's *_coeff commands set coefficients
+            throw Error("data file: section '" + t + "' is not supported (use pair_coeff/bond_coeff in run.in)");
+        }
+    }
+    if (!got_masses) throw Error("Masses section is missing.");
+    if (!got_atoms) throw Error("Atoms section is missing.");
+    for (double m : s.masses)
+        if (!(m > 0.0)) throw Error("All the masses are not set.");
+    if (s.atom_style == 1 && s.n_bonds > 0 && !got_bonds) throw Error("Bonds section is missing.");
+    for (size_t k = 0; k < N; ++k)
+        if (s.h_id[k] == 0) throw Error("data file: atom " + std::to_string(k + 1) + " is missing");
+}
+
+void write_data_host(const System &s, const std::string &path)
+{
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) throw Error("Error opening file: " + path);
+    const int N = s.n_atoms;
+    const bool bonded = s.atom_style == 1;
+    std::fprintf(f, "# Lammps format data file\n\n");
+    std::fprintf(f, "%d atoms\n", N);
+    if (bonded) std::fprintf(f, "%d bonds\n", s.n_bonds);
+    std::fprintf(f, "%d atom types\n", s.n_atomtypes);
+    if (bonded) std::fprintf(f, "%d bond types\n", s.n_bondtypes);
+    std::fprintf(f, "\n%.9f %.9f xlo xhi\n%.9f %.9f ylo yhi\n%.9f %.9f zlo zhi\n\n", s.box.xlo, s.box.xhi,
+                 s.box.ylo, s.box.yhi, s.box.zlo, s.box.zhi);
+    std::fprintf(f, "Masses\n\n");
+    for (int t = 0; t < s.n_atomtypes; ++t) std::fprintf(f, "%d %.9f\n", t + 1, s.masses[(size_t)t]);
+    std::fprintf(f, "\nAtoms  # %s\n\n", bonded ? "bond" : "atomic");
+    for (int k = 0; k < N; ++k) {
+        const double *x = &s.h_pos[3 * (size_t)k];
+        if (bonded)
+            std::fprintf(f, "%d %d %d %.9f %.9f %.9f\n", s.h_id[k],
+                         s.h_mol_id.empty() ? 0 : s.h_mol_id[(size_t)s.h_id[k] - 1], s.h_type[k], x[0], x[1], x[2]);
+        else
+            std::fprintf(f, "%d %d %.9f %.9f %.9f\n", s.h_id[k], s.h_type[k], x[0], x[1], x[2]);
+    }
+    std::fprintf(f, "\nVelocities\n\n");
+    for (int k = 0; k < N; ++k) {
+        const double *v = s.h_vel.empty() ? nullptr : &s.h_vel[3 * (size_t)k];
+        std::fprintf(f, "%d %.9f %.9f %.9f\n", s.h_id[k], v ? v[0] : 0.0, v ? v[1] : 0.0, v ? v[2] : 0.0);
+    }
+    if (bonded) {
+        std::fprintf(f, "\nBonds\n\n");
+        for (int b = 0; b < s.n_bonds; ++b)
+            std::fprintf(f, "%d %d %d %d\n", b + 1, s.h_bondlist[3 * (size_t)b], s.h_bondlist[3 * (size_t)b + 1],
+                         s.h_bondlist[3 * (size_t)b + 2]);
+    }
+    std::fclose(f);
+}
+
+void write_data(System &s, const std::string &path)
+{
+    atoms_from_device(s);
+    write_data_host(s, path);
 }
 
 void atoms_to_device(System &s, int device)
@@ -145,13 +414,18 @@ void atoms_to_device(System &s, int device)
 
 void atoms_from_device(System &s, std::vector<double> *force)
 {
-    s.h_pos.resize(3 * (size_t)s.n_atoms);
-    s.h_vel.resize(3 * (size_t)s.n_atoms);
-    if (force) force->resize(3 * (size_t)s.n_atoms);
+    const size_t N = (size_t)s.n_atoms;
+    s.h_pos.resize(3 * N);
+    s.h_vel.resize(3 * N);
+    if (force) force->resize(3 * N);
     check(s, gdpd_download(s.ctx, s.handle, s.h_pos.data(), s.h_vel.data(), nullptr,
                            force ? force->data() : nullptr));
-    // host arrays are id-ordered after a download
-    for (int i = 0; i < s.n_atoms; ++i) s.h_id[i] = i + 1;
+    // host arrays are id-ordered after a download: types follow their ids
+    std::vector<int> type_by_id(N, 0);
+    for (size_t i = 0; i < N; ++i)
+        if (s.h_id[i] >= 1 && (size_t)s.h_id[i] <= N) type_by_id[(size_t)s.h_id[i] - 1] = s.h_type[i];
+    s.h_type = type_by_id;
+    for (size_t i = 0; i < N; ++i) s.h_id[i] = (int)i + 1;
 }
 
 void release(System &s)
@@ -524,6 +798,42 @@ void simulation(System &system, Neigh_list &nl, Thermo &thermo)
         Fix_end_of_step(system, step);
         thermo.process(system, step);
     }
+}
+
+void run_deck(System &system, Thermo &thermo, const std::string &deck_text, const std::string &dir, bool fused,
+              int device)
+{
+    read_input(system, deck_text);
+    if (system.run_p.input_data_name.empty()) throw Error("read_data command is missing");
+    auto resolve = [&](const std::string &name) {
+        return (name.empty() || name[0] == '/' || dir.empty()) ? name : dir + "/" + name;
+    };
+    read_data(system, resolve(system.run_p.input_data_name));
+    // fix nve integrates every bead: its group must cover all atom types
+    std::map<std::string, std::vector<int>> groups;
+    for_lines(system.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
+        if (key != "group") return;
+        if (a.size() < 3 || a[1] != "type") illegal(l, "illegal group command", "group name type t1 t2 ...");
+        for (size_t k = 2; k < a.size(); ++k) groups[a[0]].push_back((int)to_i(a[k], l, "type", "group name type t1 ..."));
+    });
+    for_lines(system.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
+        if (key != "fix" || a.size() < 2 || a[1] == "all") return;
+        auto g = groups.find(a[1]);
+        if (g == groups.end()) illegal(l, "unknown group " + a[1], "group name type t1 t2 ...");
+        for (int t = 1; t <= system.n_atomtypes; ++t)
+            if (std::find(g->second.begin(), g->second.end(), t) == g->second.end())
+                throw Error("fix " + a[0] + ": groups that leave out bead types are not supported by this build");
+    });
+    atoms_to_device(system, device);
+    Neigh_list nl;
+    preprocess(system, nl, thermo);
+    if (fused) {
+        run_fused(system, thermo);
+    } else {
+        zero_step(system, nl, thermo);
+        simulation(system, nl, thermo);
+    }
+    if (system.run_p.write_data_flag) write_data(system, resolve(system.run_p.out_data_name));
 }
 
 void run_fused(System &system, Thermo &thermo)

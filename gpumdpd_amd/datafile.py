@@ -1,0 +1,145 @@
+"""LAMMPS-style data files and run.in decks for the configs in configs.py.
+
+The layout is the one the reference's read_data/write_data use
+(system/read_data.cu:201-381, 469-637): header counts, box, Masses, Atoms
+(atomic: id type x y z; bond: id mol type x y z), Velocities, Bonds.  These
+helpers let a test or a user turn a generated system into the files a
+reference deck reads, and read back a file written by write_data.
+"""
+import numpy as np
+
+
+def write_data(cfg, path, title="gdpd data file"):
+    """Write cfg (a configs.* dictionary) as a data file; atom_style bond when
+    cfg has bonds.  Rows are written in id order."""
+    n = cfg["N"]
+    order = np.argsort(cfg["id"])
+    ids = np.asarray(cfg["id"])[order]
+    types = np.asarray(cfg["type"])[order]
+    pos = np.asarray(cfg["pos"])[order]
+    vel = np.asarray(cfg["vel"])[order]
+    bonded = "bonds" in cfg
+    lines = [f"# {title}", "", f"{n} atoms"]
+    if bonded:
+        lines.append(f"{len(cfg['bonds'])} bonds")
+    lines.append(f"{cfg['ntypes']} atom types")
+    if bonded:
+        lines.append(f"{cfg['n_bond_types']} bond types")
+    lo, hi = cfg["lo"], cfg["hi"]
+    lines += ["", f"{lo[0]:.17g} {hi[0]:.17g} xlo xhi", f"{lo[1]:.17g} {hi[1]:.17g} ylo yhi",
+              f"{lo[2]:.17g} {hi[2]:.17g} zlo zhi", "", "Masses", ""]
+    lines += [f"{t + 1} {m:.17g}" for t, m in enumerate(cfg["masses"])]
+    lines += ["", "Atoms  # " + ("bond" if bonded else "atomic"), ""]
+    mol = cfg.get("mol")
+    for k in range(n):
+        x = pos[k]
+        if bonded:
+            m = int(mol[ids[k] - 1]) if mol is not None else 1 + (ids[k] - 1) // 16
+            lines.append(f"{ids[k]} {m} {types[k]} {x[0]:.17g} {x[1]:.17g} {x[2]:.17g}")
+        else:
+            lines.append(f"{ids[k]} {types[k]} {x[0]:.17g} {x[1]:.17g} {x[2]:.17g}")
+    lines += ["", "Velocities", ""]
+    lines += [f"{ids[k]} {vel[k][0]:.17g} {vel[k][1]:.17g} {vel[k][2]:.17g}" for k in range(n)]
+    if bonded:
+        lines += ["", "Bonds", ""]
+        b = np.asarray(cfg["bonds"])
+        lines += [f"{i + 1} {int(b[i, 0])} {int(b[i, 1])} {int(b[i, 2])}" for i in range(len(b))]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_data(path):
+    """Parse a data file into id-ordered arrays: dict with n, types, box,
+    masses, id, type, mol (or None), pos, vel (or None), bonds (or None)."""
+    with open(path) as f:
+        raw = f.read().split("\n")
+    strip = [l.split("#")[0].strip() for l in raw]
+    out = {"mol": None, "vel": None, "bonds": None}
+    i = 1
+    counts = {}
+    sections = {"Masses", "Atoms", "Velocities", "Bonds"}
+    lo, hi = [0.0] * 3, [0.0] * 3
+    while i < len(strip) and strip[i] not in sections:
+        t = strip[i].split()
+        if len(t) >= 2 and t[-1] in ("atoms", "bonds"):
+            counts[t[-1]] = int(t[0])
+        elif len(t) >= 3 and t[-1] == "types":
+            counts[t[-2] + " types"] = int(t[0])
+        elif len(t) == 4 and t[2].endswith("lo"):
+            d = "xyz".index(t[2][0])
+            lo[d], hi[d] = float(t[0]), float(t[1])
+        i += 1
+    n = counts["atoms"]
+    out.update(n=n, lo=lo, hi=hi, ntypes=counts["atom types"])
+    style_bond = False
+
+    def rows(start, count):
+        r, k = [], start
+        while len(r) < count:
+            if strip[k]:
+                r.append(strip[k].split())
+            k += 1
+        return r, k
+
+    while i < len(strip):
+        name = strip[i]
+        if name not in sections:
+            i += 1
+            continue
+        if name == "Atoms":
+            style_bond = "bond" in raw[i]
+        count = {"Masses": out["ntypes"], "Atoms": n, "Velocities": n, "Bonds": counts.get("bonds", 0)}[name]
+        r, i = rows(i + 1, count)
+        a = np.array(r, dtype=np.float64) if r else np.zeros((0, 4))
+        if name == "Masses":
+            out["masses"] = a[np.argsort(a[:, 0]), 1]
+        elif name == "Atoms":
+            a = a[np.argsort(a[:, 0])]
+            out["id"] = a[:, 0].astype(np.int64)
+            if style_bond:
+                out["mol"], out["type"], out["pos"] = a[:, 1].astype(np.int64), a[:, 2].astype(np.int64), a[:, 3:6]
+            else:
+                out["type"], out["pos"] = a[:, 1].astype(np.int64), a[:, 2:5]
+        elif name == "Velocities":
+            a = a[np.argsort(a[:, 0])]
+            out["vel"] = a[:, 1:4]
+        elif name == "Bonds":
+            a = a[np.argsort(a[:, 0])]
+            out["bonds"] = a[:, 1:4].astype(np.int64)
+    return out
+
+
+def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None):
+    """A reference-style run.in for cfg (pair style, coefficients, bonds,
+    neighbor, timestep, thermo, run, read_data, group and fix nve)."""
+    bonded = "bonds" in cfg
+    nt = cfg["ntypes"]
+    lines = ["# generated deck", f"atom_style {'bond' if bonded else 'atomic'}",
+             f"neighbor {cfg['global_cut']} {cfg['skin']} {cfg['max_rho']} 1", f"timestep {cfg['dt']}",
+             f"thermo {log_every}", f"run {nsteps}", f"read_data {data_name}"]
+    if write_name:
+        lines.append(f"write_data {write_name}")
+    c = np.asarray(cfg["coeff"], dtype=np.float64)
+    if cfg["style"] == "mdpd":
+        lines.append(f"pair_style mdpd {cfg['temp']} {cfg['rc']} {cfg['rd']} {cfg['saru_seed']}")
+        w = 5
+    else:
+        lines.append(f"pair_style dpd {cfg['temp']} {cfg['rc']} {cfg['saru_seed']}")
+        w = 3
+    c = c.reshape(-1, w)
+    k = 0
+    for i in range(1, nt + 1):
+        for j in range(i, nt + 1):
+            v = c[k]
+            if w == 3:
+                lines.append(f"pair_coeff {i} {j} {v[0]:.17g} {v[1]:.17g} {v[2]:.17g}")
+            else:
+                lines.append(f"pair_coeff {i} {j} {v[0]:.17g} {v[1]:.17g} {v[2]:.17g} {v[3]:.17g} {v[4]:.17g}")
+            k += 1
+    if bonded:
+        lines.append("bond_style harmonic")
+        bc = np.asarray(cfg["bond_coeff"], dtype=np.float64).reshape(-1, 2)
+        lines += [f"bond_coeff {t + 1} {bc[t, 0]:.17g} {bc[t, 1]:.17g}" for t in range(len(bc))]
+    lines += ["group all type " + " ".join(str(t) for t in range(1, nt + 1)), "fix md all nve"]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

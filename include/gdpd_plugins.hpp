@@ -30,6 +30,8 @@ struct Run_p {
     int nl_f = 1;
     double dt = 0.01;
     unsigned log_f = 100, nsteps = 0;
+    std::string input_data_name, out_data_name;  // read_data / write_data
+    bool write_data_flag = false;
 };
 
 // system/system.cuh:53-59
@@ -53,7 +55,9 @@ struct System {
     Run_p run_p;
     Thermo_p thermo_p;
     std::vector<int> h_id, h_type, h_bondlist;
+    std::vector<int> h_mol_id;    // molecule ids (atom_style bond), index = id - 1
     std::vector<double> h_pos, h_vel, masses;
+    bool has_velocities = false;  // the data file had a Velocities section
     int precision = GDPD_PREC_F32;
     int h_update_flag = 0;        // Neigh::h_update_flag
     gdpd_ctx *ctx = nullptr;
@@ -67,6 +71,17 @@ void read_input(System &s, const std::string &text);
 // atoms_mem_alloc + atoms_mem_copy_2_gpu (system/system.cu:5-86): creates the
 // device system from the host arrays and uploads them
 void atoms_to_device(System &s, int device = 0);
+// Read_data::read_data (system/read_data.cu:5-9, 201-381): a LAMMPS-style data
+// file with the header counts, box, Masses, Atoms (atomic: id type x y z;
+// bond: id mol type x y z), optional Velocities and, for atom_style bond,
+// Bonds.  Fills the host arrays id-ordered (index = id - 1); throws Error
+// with the reference's messages on malformed input.
+void read_data(System &s, const std::string &path);
+// Read_data::write_data (system/read_data.cu:469-637) from the host arrays
+// (write_data_host) or from the device state (write_data), in the layout the
+// reference writes, so a written file reads back with read_data.
+void write_data_host(const System &s, const std::string &path);
+void write_data(System &s, const std::string &path);
 void atoms_from_device(System &s, std::vector<double> *force = nullptr);
 void release(System &s);
 
@@ -206,5 +221,12 @@ void preprocess(System &system, Neigh_list &nl, Thermo &thermo);
 void zero_step(System &system, Neigh_list &nl, Thermo &thermo);
 void simulation(System &system, Neigh_list &nl, Thermo &thermo);
 void run_fused(System &system, Thermo &thermo);
+
+// main/main.cu:15-72 for a deck: read_input, read_data (a relative file name is
+// taken relative to dir), the device upload, preprocess, the run (fused, or
+// one call per hook) and write_data when the deck asks for it.  The system
+// keeps its device state for the caller; release() frees it.
+void run_deck(System &system, Thermo &thermo, const std::string &deck_text, const std::string &dir,
+              bool fused, int device = 0);
 
 } // namespace gdpd_host

@@ -1,0 +1,128 @@
+"""Data-file front end: the shells' read_data/write_data (C++) against the
+Python writer and reader, the reference's error cases, and a whole deck
+(run.in + data file) run through the C++ shells on the GPU, compared with the
+same system run through the C ABI directly."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gpumdpd_amd as g
+from gpumdpd_amd import configs, datafile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROUNDTRIP = os.path.join(ROOT, "tests", "cpp", "data_roundtrip")
+RUNNER = os.path.join(ROOT, "tests", "cpp", "deck_runner")
+
+
+def roundtrip(style, src, dst):
+    assert os.path.exists(ROUNDTRIP), "build() compiles tests/cpp/data_roundtrip"
+    return subprocess.run([ROUNDTRIP, style, src, dst], capture_output=True, text=True, timeout=60)
+
+
+def bonded_cfg():
+    cfg = configs.config_c2(n_chains=60, box=8.0)
+    cfg["mol"] = 1 + (np.arange(cfg["N"]) // 16)
+    return cfg
+
+
+def compare(d, cfg, tol):
+    order = np.argsort(cfg["id"])
+    assert d["n"] == cfg["N"]
+    assert np.array_equal(d["id"], np.arange(1, cfg["N"] + 1))
+    assert np.array_equal(d["type"], np.asarray(cfg["type"])[order])
+    assert np.allclose(d["pos"], np.asarray(cfg["pos"])[order], atol=tol, rtol=0)
+    assert np.allclose(d["vel"], np.asarray(cfg["vel"])[order], atol=tol, rtol=0)
+    assert np.allclose(d["masses"], cfg["masses"], atol=tol)
+    assert np.allclose(d["lo"], cfg["lo"]) and np.allclose(d["hi"], cfg["hi"])
+
+
+def test_roundtrip_atomic(tmp_path):
+    cfg = configs.scaled_dpd(2000, seed=3)
+    src, dst = str(tmp_path / "in.data"), str(tmp_path / "out.data")
+    datafile.write_data(cfg, src)
+    r = roundtrip("atomic", src, dst)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split() == ["ok", str(cfg["N"]), "0", "1"]
+    compare(datafile.read_data(dst), cfg, 6e-10)
+
+
+def test_roundtrip_bonded_keeps_molecules_and_bonds(tmp_path):
+    cfg = bonded_cfg()
+    src, dst = str(tmp_path / "in.data"), str(tmp_path / "out.data")
+    datafile.write_data(cfg, src)
+    r = roundtrip("bond", src, dst)
+    assert r.returncode == 0, r.stderr
+    d = datafile.read_data(dst)
+    compare(d, cfg, 6e-10)
+    assert np.array_equal(d["mol"], cfg["mol"])
+    assert np.array_equal(d["bonds"], np.asarray(cfg["bonds"]))
+
+
+def test_positions_outside_the_box_are_wrapped(tmp_path):
+    cfg = configs.scaled_dpd(500, seed=4)
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    shifted = dict(cfg, pos=cfg["pos"] + np.array([2.0, -1.0, 3.0]) * L)
+    src, dst = str(tmp_path / "in.data"), str(tmp_path / "out.data")
+    datafile.write_data(shifted, src)
+    assert roundtrip("atomic", src, dst).returncode == 0
+    compare(datafile.read_data(dst), cfg, 1e-8)
+
+
+@pytest.mark.parametrize("edit,message", [
+    (lambda t: t.replace("Masses", "Massless"), "Massless"),
+    (lambda t: t.replace("Atoms  # atomic", "Atoms  # bond"), "inconsistent"),
+    (lambda t: t.replace("\n2 1 ", "\n1 1 ", 1), "duplicate atom id"),
+    (lambda t: t.replace("\n1 1 ", "\n0 1 ", 1), "Atom ID cannot be 0"),
+    (lambda t: t.replace(" atom types", " atom types\n3 angles"), "angle"),
+])
+def test_reference_error_cases(tmp_path, edit, message):
+    cfg = configs.scaled_dpd(300, seed=5)
+    src = str(tmp_path / "in.data")
+    datafile.write_data(cfg, src)
+    text = open(src).read()
+    bad = edit(text)
+    assert bad != text
+    open(src, "w").write(bad)
+    r = roundtrip("atomic", src, str(tmp_path / "out.data"))
+    assert r.returncode == 1
+    assert message in r.stderr
+
+
+def run_deck(tmp_path, cfg, nsteps, log_every, mode, prec):
+    data, deck, out = tmp_path / "in.data", tmp_path / "run.in", tmp_path / f"out_{mode}.data"
+    datafile.write_data(cfg, str(data))
+    datafile.write_deck(str(deck), cfg, "in.data", nsteps, log_every, write_name=out.name)
+    assert os.path.exists(RUNNER), "build() compiles tests/cpp/deck_runner"
+    r = subprocess.run([RUNNER, str(deck), mode, prec], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = np.array([list(map(float, l.split()[1:])) for l in r.stdout.splitlines() if l.startswith("thermo")])
+    return datafile.read_data(str(out)), rows
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["dpd", "bonded"])
+def test_deck_through_shells_matches_direct_run(tmp_path, name):
+    cfg = configs.scaled_dpd(3000, seed=6) if name == "dpd" else bonded_cfg()
+    nsteps, every = 20, 5
+    print(f"case: deck {name}", flush=True)
+    d_fused, rows_fused = run_deck(tmp_path, cfg, nsteps, every, "fused", "f64")
+    d_hook, rows_hook = run_deck(tmp_path, cfg, nsteps, every, "hook", "f64")
+    ctx = g.Context(0)
+    try:
+        s = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=every)
+        s.run(nsteps)
+        ref = s.download()
+        ref_rows = s.thermo_rows()
+        s.destroy()
+    finally:
+        ctx.close()
+    for d in (d_fused, d_hook):
+        assert np.allclose(d["pos"], ref["pos"], atol=6e-10, rtol=0)
+        assert np.allclose(d["vel"], ref["vel"], atol=6e-10, rtol=0)
+    # step, T, pair PE, bond PE, P of every thermo row
+    assert rows_fused.shape == (nsteps // every + 1, 5)
+    assert np.allclose(rows_fused[:, 0], ref_rows[:, 11])
+    assert np.allclose(rows_fused[:, 1:], ref_rows[:, [0, 1, 2, 3]], rtol=1e-12, atol=1e-12)
+    assert np.allclose(rows_hook, rows_fused, rtol=1e-11, atol=1e-11)
