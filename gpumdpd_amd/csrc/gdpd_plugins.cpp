@@ -123,8 +123,18 @@ void read_input(System &s, const std::string &text)
             s.run_p.out_data_name = a[0];
             s.run_p.write_data_flag = true;
         } else if (key == "dump") {
-            // trajectory output is not part of this library
-            std::fprintf(stderr, "gdpd: ignoring dump command (no trajectory output): %s\n", l.c_str());
+            const char *fmt = "dump start_step frequency wrapped/unwrapped force_yes/force_no vel_yes/vel_no";
+            if (a.size() != 5) illegal(l, "illegal dump command", fmt);
+            s.run_p.dump_start = (unsigned)to_i(a[0], l, "start_step", fmt);
+            s.run_p.dump_f = (unsigned)to_i(a[1], l, "frequency", fmt);
+            if (s.run_p.dump_f == 0) illegal(l, "illegal dump command", fmt);
+            if (a[2] != "wrapped" && a[2] != "unwrapped") illegal(l, "illegal dump command", fmt);
+            if (a[3] != "force_yes" && a[3] != "force_no") illegal(l, "illegal dump command", fmt);
+            if (a[4] != "vel_yes" && a[4] != "vel_no") illegal(l, "illegal dump command", fmt);
+            s.run_p.wrapped_flag = a[2] == "wrapped";
+            s.run_p.dumpforce_flag = a[3] == "force_yes";
+            s.run_p.dumpvel_flag = a[4] == "vel_yes";
+            s.run_p.write_dump_flag = true;
         } else if (key == "angle_style" || key == "angle_coeff" || key == "compute") {
             throw Error("\n   Error: " + key + " is not supported by this build\n   Line: " + l);
         } else if (key != "pair_style" && key != "pair_coeff" && key != "bond_style" && key != "bond_coeff" &&
@@ -739,8 +749,38 @@ void Neigh_list::check_update(System &s) { check(s, gdpd_neigh_check(s.ctx, s.ha
 
 // ---- Thermo (compute/thermo.cu:96-220) -------------------------------------------
 
-void Thermo::preprocess(System &) { rows.clear(); }
-void Thermo::postprocess(System &) {}
+void Thermo::preprocess(System &s)
+{
+    rows.clear();
+    if (file_path.empty()) return;
+    file = std::fopen(file_path.c_str(), "w");
+    if (!file) throw Error("cannot open thermo file: " + file_path);
+    if (s.atom_style == 1)
+        std::fprintf(file, "%15s\t%10s\t%20s\t%20s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\n", "step", "temp",
+                     "pair_pe", "bond_pe", "pressure", "pxx", "pxy", "pxz", "pyy", "pyz", "pzz");
+    else
+        std::fprintf(file, "%15s\t%10s\t%20s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\n", "step", "temp",
+                     "pair_pe", "pressure", "pxx", "pxy", "pxz", "pyy", "pyz", "pzz");
+}
+
+void Thermo::postprocess(System &)
+{
+    if (file) std::fclose(file);
+    file = nullptr;
+}
+
+void Thermo::write_row(const System &s, const std::vector<double> &r)
+{
+    // r: step, T, pair PE, bond PE, P, Pxx Pxy Pxz Pyy Pyz Pzz
+    if (!file) return;
+    if (s.atom_style == 1)
+        std::fprintf(file, "%15u\t%10.6f\t%20.6f\t%20.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\n",
+                     (unsigned)r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]);
+    else
+        std::fprintf(file, "%15u\t%10.6f\t%20.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\n",
+                     (unsigned)r[0], r[1], r[2], r[4], r[5], r[6], r[7], r[8], r[9], r[10]);
+    std::fflush(file);
+}
 
 void Thermo::process(System &s, unsigned int step)
 {
@@ -754,6 +794,7 @@ void Thermo::process(System &s, unsigned int step)
     for (int k = 0; k < 6; ++k) s.thermo_p.pressure_tensor[k] = o[GDPD_THERMO_PXX + k];
     std::vector<double> r = {(double)step, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9]};
     rows.push_back(r);
+    write_row(s, r);
     if (print)
         std::printf("%15u\t%10.6f\t%20.6f\t%20.6f\t%12.6f\n", step, o[0], o[1], o[2], o[3]);
 }
@@ -769,16 +810,61 @@ void preprocess(System &system, Neigh_list &nl, Thermo &thermo)
     thermo.preprocess(system);
 }
 
-void zero_step(System &system, Neigh_list &nl, Thermo &thermo)
+// ---- Dump (system/dump.cu) ------------------------------------------------------------
+
+void Dump::preprocess(System &, const std::string &path)
+{
+    file = std::fopen(path.c_str(), "w");
+    if (!file) throw Error("cannot open dump file: " + path);
+}
+
+void Dump::postprocess(System &)
+{
+    if (file) std::fclose(file);
+    file = nullptr;
+}
+
+void Dump::dump(System &s, unsigned int step)
+{
+    if (!file) return;
+    const size_t N = (size_t)s.n_atoms;
+    std::vector<double> pos(3 * N), vel(3 * N), force(3 * N);
+    std::vector<int> image(3 * N);
+    check(s, gdpd_download(s.ctx, s.handle, pos.data(), vel.data(), image.data(), force.data()));
+    // types by id (host arrays may be in upload order)
+    std::vector<int> type_by_id(N, 0);
+    for (size_t i = 0; i < N; ++i)
+        if (s.h_id[i] >= 1 && (size_t)s.h_id[i] <= N) type_by_id[(size_t)s.h_id[i] - 1] = s.h_type[i];
+    const Run_p &rp = s.run_p;
+    const double L[3] = {s.box.xhi - s.box.xlo, s.box.yhi - s.box.ylo, s.box.zhi - s.box.zlo};
+    std::fprintf(file, "ITEM: TIMESTEP\n%u\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n", step, s.n_atoms);
+    std::fprintf(file, "%.9f %.9f\n%.9f %.9f\n%.9f %.9f\n", s.box.xlo, s.box.xhi, s.box.ylo, s.box.yhi, s.box.zlo,
+                 s.box.zhi);
+    std::fprintf(file, "ITEM: ATOMS id type x y z%s%s\n", rp.dumpforce_flag ? " fx fy fz" : "",
+                 rp.dumpvel_flag ? " vx vy vz" : "");
+    for (size_t i = 0; i < N; ++i) {
+        double x[3];
+        for (int d = 0; d < 3; ++d) x[d] = pos[3 * i + d] + (rp.wrapped_flag ? 0.0 : image[3 * i + d] * L[d]);
+        std::fprintf(file, "%d %d %.9f %.9f %.9f", (int)i + 1, type_by_id[i], x[0], x[1], x[2]);
+        if (rp.dumpforce_flag) std::fprintf(file, " %.9f %.9f %.9f", force[3 * i], force[3 * i + 1], force[3 * i + 2]);
+        if (rp.dumpvel_flag) std::fprintf(file, " %.9f %.9f %.9f", vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+        std::fprintf(file, "\n");
+    }
+    std::fflush(file);
+}
+
+void zero_step(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump)
 {
     nl.build(system);
     Pair_compute(system, 0);
     if (system.atom_style == 1) Bond_compute(system, 0);
     Fix_post_force(system, 0);
     thermo.process(system, 0);
+    // main/utils.cuh:200-204
+    if (dump && system.run_p.write_dump_flag && system.run_p.dump_start == 0) dump->dump(system, 0);
 }
 
-void simulation(System &system, Neigh_list &nl, Thermo &thermo)
+void simulation(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump)
 {
     for (unsigned int step = 1; step <= system.run_p.nsteps; ++step) {
         Fix_initial_integrate(system, step);
@@ -797,6 +883,10 @@ void simulation(System &system, Neigh_list &nl, Thermo &thermo)
         Fix_final_integrate(system, step);
         Fix_end_of_step(system, step);
         thermo.process(system, step);
+        // main/utils.cuh:296-300
+        const Run_p &rp = system.run_p;
+        if (dump && rp.write_dump_flag && step >= rp.dump_start && (step - rp.dump_start) % rp.dump_f == 0)
+            dump->dump(system, step);
     }
 }
 
@@ -826,13 +916,19 @@ void run_deck(System &system, Thermo &thermo, const std::string &deck_text, cons
     });
     atoms_to_device(system, device);
     Neigh_list nl;
+    if (thermo.file_path.empty()) thermo.file_path = resolve("thermo.txt");
     preprocess(system, nl, thermo);
-    if (fused) {
+    if (fused && !system.run_p.write_dump_flag) {
         run_fused(system, thermo);
+        for (const auto &r : thermo.rows) thermo.write_row(system, r);
     } else {
-        zero_step(system, nl, thermo);
-        simulation(system, nl, thermo);
+        Dump dump;
+        if (system.run_p.write_dump_flag) dump.preprocess(system, resolve("all.trj"));
+        zero_step(system, nl, thermo, &dump);
+        simulation(system, nl, thermo, &dump);
+        dump.postprocess(system);
     }
+    thermo.postprocess(system);
     if (system.run_p.write_data_flag) write_data(system, resolve(system.run_p.out_data_name));
 }
 

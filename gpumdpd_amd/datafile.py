@@ -109,9 +109,10 @@ def read_data(path):
     return out
 
 
-def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None):
+def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None, dump=None):
     """A reference-style run.in for cfg (pair style, coefficients, bonds,
-    neighbor, timestep, thermo, run, read_data, group and fix nve)."""
+    neighbor, timestep, thermo, run, read_data, group and fix nve); dump is
+    the argument string of a dump command, e.g. "0 5 wrapped force_yes vel_yes"."""
     bonded = "bonds" in cfg
     nt = cfg["ntypes"]
     lines = ["# generated deck", f"atom_style {'bond' if bonded else 'atomic'}",
@@ -119,6 +120,8 @@ def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None):
              f"thermo {log_every}", f"run {nsteps}", f"read_data {data_name}"]
     if write_name:
         lines.append(f"write_data {write_name}")
+    if dump:
+        lines.append(f"dump {dump}")
     c = np.asarray(cfg["coeff"], dtype=np.float64)
     if cfg["style"] == "mdpd":
         lines.append(f"pair_style mdpd {cfg['temp']} {cfg['rc']} {cfg['rd']} {cfg['saru_seed']}")
@@ -143,3 +146,30 @@ def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None):
     lines += ["group all type " + " ".join(str(t) for t in range(1, nt + 1)), "fix md all nve"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def read_dump(path):
+    """Frames of a LAMMPS text dump: list of (step, columns, array rows x cols)."""
+    frames = []
+    with open(path) as f:
+        lines = f.read().split("\n")
+    i = 0
+    while i < len(lines):
+        if lines[i].startswith("ITEM: TIMESTEP"):
+            step = int(lines[i + 1])
+            n = int(lines[i + 3])
+            cols = lines[i + 8].split()[2:]
+            data = np.array([list(map(float, lines[i + 9 + k].split())) for k in range(n)])
+            frames.append((step, cols, data))
+            i += 9 + n
+        else:
+            i += 1
+    return frames
+
+
+def read_thermo(path):
+    """thermo.txt written by the deck runner: (header columns, rows)."""
+    with open(path) as f:
+        head = f.readline().split()
+        rows = np.array([list(map(float, l.split())) for l in f if l.strip()])
+    return head, rows

@@ -32,6 +32,9 @@ struct Run_p {
     unsigned log_f = 100, nsteps = 0;
     std::string input_data_name, out_data_name;  // read_data / write_data
     bool write_data_flag = false;
+    // dump start_step frequency wrapped|unwrapped force_yes|no vel_yes|no
+    bool write_dump_flag = false, wrapped_flag = true, dumpforce_flag = false, dumpvel_flag = false;
+    unsigned dump_start = 0, dump_f = 1;
 };
 
 // system/system.cuh:53-59
@@ -205,27 +208,46 @@ public:
 };
 
 // ---- Thermo (compute/thermo.cuh:5-19) ----------------------------------------
+// rows keeps every record; with file_path set, the records also go to that
+// file in the reference's thermo.txt columns (compute/thermo.cu:41-62, 122-214)
 class Thermo {
 public:
     void preprocess(System &system);
     void process(System &system, unsigned int step);
     void postprocess(System &system);
+    void write_row(const System &system, const std::vector<double> &row);
     std::vector<std::vector<double>> rows;  // step, T, pair pe, bond pe, P, P6
     bool print = false;
+    std::string file_path;
+    FILE *file = nullptr;
+};
+
+// ---- Dump (system/dump.cu) ---------------------------------------------------
+// LAMMPS text dump frames (ITEM: TIMESTEP / NUMBER OF ATOMS / BOX BOUNDS pp pp
+// pp / ATOMS id type x y z [fx fy fz] [vx vy vz]), id-ordered, positions
+// wrapped or unwrapped (x + image * L) as the dump command asks
+class Dump {
+public:
+    void preprocess(System &system, const std::string &path = "all.trj");
+    void dump(System &system, unsigned int step);
+    void postprocess(System &system);
+    FILE *file = nullptr;
 };
 
 // The reference's driver (main/utils.cuh:126-314) over the shells: preprocess,
 // zero_step, simulation and postprocess.  fused = true runs `run N` through
 // gdpd_run (one launch per step) instead of one call per hook.
 void preprocess(System &system, Neigh_list &nl, Thermo &thermo);
-void zero_step(System &system, Neigh_list &nl, Thermo &thermo);
-void simulation(System &system, Neigh_list &nl, Thermo &thermo);
+void zero_step(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump = nullptr);
+void simulation(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump = nullptr);
 void run_fused(System &system, Thermo &thermo);
 
 // main/main.cu:15-72 for a deck: read_input, read_data (a relative file name is
 // taken relative to dir), the device upload, preprocess, the run (fused, or
-// one call per hook) and write_data when the deck asks for it.  The system
-// keeps its device state for the caller; release() frees it.
+// one call per hook) and write_data when the deck asks for it.  thermo.txt and
+// the dump file all.trj are written into dir; a deck with a dump command runs
+// through the hook path (dumps need the state at the end of a step).  The
+// system keeps its device state for the caller; release() frees it.
 void run_deck(System &system, Thermo &thermo, const std::string &deck_text, const std::string &dir,
               bool fused, int device = 0);
 

@@ -126,3 +126,44 @@ def test_deck_through_shells_matches_direct_run(tmp_path, name):
     assert np.allclose(rows_fused[:, 0], ref_rows[:, 11])
     assert np.allclose(rows_fused[:, 1:], ref_rows[:, [0, 1, 2, 3]], rtol=1e-12, atol=1e-12)
     assert np.allclose(rows_hook, rows_fused, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wrapped", [True, False])
+def test_deck_dump_and_thermo_files(tmp_path, wrapped):
+    """dump frames (main/utils.cuh:200-204, 296-300 timing; system/dump.cu
+    columns) and thermo.txt (compute/thermo.cu columns) against a direct run."""
+    cfg = configs.scaled_dpd(2500, seed=8)
+    nsteps, every = 12, 4
+    data, deck = tmp_path / "in.data", tmp_path / "run.in"
+    datafile.write_data(cfg, str(data))
+    mode = "wrapped" if wrapped else "unwrapped"
+    datafile.write_deck(str(deck), cfg, "in.data", nsteps, every, dump=f"2 5 {mode} force_yes vel_yes")
+    print(f"case: deck dump {mode}", flush=True)
+    r = subprocess.run([RUNNER, str(deck), "fused", "f64"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    frames = datafile.read_dump(str(tmp_path / "all.trj"))
+    assert [f[0] for f in frames] == [2, 7, 12]
+    assert frames[0][1] == ["id", "type", "x", "y", "z", "fx", "fy", "fz", "vx", "vy", "vz"]
+    head, rows = datafile.read_thermo(str(tmp_path / "thermo.txt"))
+    assert head == ["step", "temp", "pair_pe", "pressure", "pxx", "pxy", "pxz", "pyy", "pyz", "pzz"]
+    ctx = g.Context(0)
+    try:
+        s = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=every)
+        s.run(nsteps)
+        ref = s.download()
+        ref_rows = s.thermo_rows()
+        s.destroy()
+    finally:
+        ctx.close()
+    last = frames[-1][2]
+    assert np.array_equal(last[:, 0], np.arange(1, cfg["N"] + 1))
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    want = ref["pos"] if wrapped else ref["pos"] + ref["image"] * L
+    assert np.allclose(last[:, 2:5], want, atol=2e-9, rtol=0)
+    assert np.allclose(last[:, 5:8], ref["force"], atol=2e-9, rtol=0)
+    assert np.allclose(last[:, 8:11], ref["vel"], atol=2e-9, rtol=0)
+    assert np.allclose(rows[:, 0], ref_rows[:, 11])
+    assert np.allclose(rows[:, 1], ref_rows[:, 0], atol=2e-6)
+    assert np.allclose(rows[:, 2], ref_rows[:, 1], atol=2e-6)
+    assert np.allclose(rows[:, 3:], ref_rows[:, 3:10], atol=2e-6)
