@@ -19,6 +19,8 @@ def system_from_config(ctx, cfg, precision=PREC_F32, log_every=100, check_every=
         s.set_pair_dpd(cfg["temp"], cfg["rc"], cfg["saru_seed"], cfg["coeff"])
     if "bonds" in cfg:
         s.set_bonds(cfg["bonds"], cfg["n_bond_types"], cfg["bond_coeff"])
+    if "angles" in cfg:
+        s.set_angles(cfg["angles"], cfg["n_angle_types"], cfg["angle_coeff"])
     if upload:
         s.upload(cfg["id"], cfg["type"], cfg["pos"], cfg["vel"], cfg["masses"])
     return s

@@ -14,7 +14,7 @@ ERR_NAMES = {1: "GDPD_ERR_ARG", 2: "GDPD_ERR_HIP", 3: "GDPD_ERR_OVERFLOW", 4: "G
              5: "GDPD_ERR_NOMEM", 6: "GDPD_ERR_NAN"}
 PREC_F32, PREC_F64 = 0, 1
 STYLE_DPD, STYLE_MDPD = 1, 2
-THERMO_LEN = 12
+THERMO_LEN = 13
 
 
 class SystemDesc(C.Structure):
@@ -46,11 +46,13 @@ PROTOTYPES = {
     "gdpd_set_pair_dpd": (_I, [_P, _I, C.c_double, C.c_double, C.c_uint32, _PD]),
     "gdpd_set_pair_mdpd": (_I, [_P, _I, C.c_double, C.c_double, C.c_double, C.c_uint32, _PD]),
     "gdpd_set_bonds": (_I, [_P, _I, _I, _I, _PI, _PD]),
+    "gdpd_set_angles": (_I, [_P, _I, _I, _I, _PI, _PD]),
     "gdpd_neigh_build": (_I, [_P, _I]),
     "gdpd_neigh_check": (_I, [_P, _I, _PI]),
     "gdpd_neigh_download": (_I, [_P, _I, _PI, _PI, _I]),
     "gdpd_pair_compute": (_I, [_P, _I, C.c_uint32, _I]),
     "gdpd_bond_compute": (_I, [_P, _I, C.c_uint32, _I]),
+    "gdpd_angle_compute": (_I, [_P, _I, C.c_uint32, _I]),
     "gdpd_nve_initial": (_I, [_P, _I]),
     "gdpd_nve_final": (_I, [_P, _I]),
     "gdpd_thermo": (_I, [_P, _I, _PD]),

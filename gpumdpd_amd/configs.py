@@ -174,3 +174,19 @@ def replica_system(style, n, seed):
                  global_cut=1.0, max_rho=8.0, coeff=mdpd_coeff(1, {(1, 1): (-40.0, 25.0, 4.5, 1.0, 0.75)}))
     s["name"] = f"{style} replica {n}"
     return s
+
+
+def config_angles(n_chains=1500, box=20.0, seed=31):
+    """Semi-flexible diblock chains: the C2 melt at a given size plus harmonic
+    angles over every consecutive triple of a chain (type 1: k=3, theta0=180
+    degrees for the A half; type 2: k=2, theta0=120 for triples touching B)."""
+    cfg = config_c2(seed=seed, n_chains=n_chains, box=box)
+    chain = 16
+    first = np.arange(n_chains)[:, None] * chain + np.arange(chain - 2)[None, :]
+    ids = first.ravel() + 1
+    t = np.where((first.ravel() % chain) + 2 < chain // 2, 1, 2)
+    cfg["angles"] = np.stack([t, ids, ids + 1, ids + 2], axis=1).astype(np.int32)
+    cfg["n_angle_types"] = 2
+    cfg["angle_coeff"] = np.array([3.0, 180.0, 2.0, 120.0])
+    cfg["name"] = f"angle melt {cfg['N']}"
+    return cfg

@@ -737,12 +737,14 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     ALLOC(h.force, 3 * (size_t)N);
     ALLOC(h.pe, N);
     ALLOC(h.bpe, N);
+    ALLOC(h.ape, N);
+    ALLOC(h.angle_start, (size_t)N + 1);
     ALLOC(h.vir, 6 * (size_t)N);
     ALLOC(h.rho, N);
     ALLOC(h.iscratch, 3 * (size_t)N);
     ALLOC(h.state, 1);
     h.thermo_cap = 1024;
-    ALLOC(h.thermo_rows, (size_t)h.thermo_cap * 12);
+    ALLOC(h.thermo_rows, (size_t)h.thermo_cap * kThermoRowLen);
     ALLOC(h.thermo_bead, (size_t)((N + kBlock - 1) / kBlock) * kThermoParts);
     // empty bond CSR (bond_start[id] == 0 for all ids)
     ALLOC(h.bond_start, (size_t)N + 1);
@@ -752,6 +754,8 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     HIPCHK(hipMemsetAsync(h.force, 0, sizeof(double) * 3 * (size_t)N, ctx->stream));
     HIPCHK(hipMemsetAsync(h.pe, 0, sizeof(double) * (size_t)N, ctx->stream));
     HIPCHK(hipMemsetAsync(h.bpe, 0, sizeof(double) * (size_t)N, ctx->stream));
+    HIPCHK(hipMemsetAsync(h.ape, 0, sizeof(double) * (size_t)N, ctx->stream));
+    HIPCHK(hipMemsetAsync(h.angle_start, 0, sizeof(int) * ((size_t)N + 1), ctx->stream));
     HIPCHK(hipMemsetAsync(h.vir, 0, sizeof(double) * 6 * (size_t)N, ctx->stream));
     HIPCHK(hipMemsetAsync(h.rho, 0, sizeof(double) * (size_t)N, ctx->stream));
     RepState st{};
@@ -999,6 +1003,56 @@ int gdpd_set_bonds(gdpd_ctx *ctx, int sid, int n_bonds, int n_bond_types, const 
     return push_desc(ctx, sid);
 }
 
+int gdpd_set_angles(gdpd_ctx *ctx, int sid, int n_angles, int n_angle_types, const int *anglelist,
+                    const double *angle_coeff)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || n_angles < 0 || (n_angles > 0 && (!anglelist || !angle_coeff)))
+        return fail(ctx, GDPD_ERR_ARG, "bad angle arguments");
+    if (n_angle_types < 0 || n_angle_types > kMaxAngleTypes)
+        return fail(ctx, GDPD_ERR_ARG, "n_angle_types must be in 0..%d", kMaxAngleTypes);
+    RepDev &h = s->h;
+    const int N = h.N;
+    // every angle is listed once for each of its three beads, with the bead's
+    // role and the other two ids in i, j, k order
+    std::vector<int> deg(N + 1, 0);
+    for (int a = 0; a < n_angles; ++a) {
+        const int t = anglelist[4 * a];
+        if (t < 1 || t > n_angle_types) return fail(ctx, GDPD_ERR_ARG, "angle %d: type exceeds n_angle_types", a);
+        for (int m = 1; m <= 3; ++m) {
+            const int id = anglelist[4 * a + m];
+            if (id < 1 || id > N) return fail(ctx, GDPD_ERR_ARG, "angle %d: atom id out of range", a);
+            deg[id]++;
+        }
+    }
+    std::vector<int> start(N + 1, 0);
+    for (int k = 0; k < N; ++k) start[k + 1] = start[k] + deg[k + 1];
+    std::vector<int> fillp(start.begin(), start.end() - 1), ent(4 * (size_t)std::max(1, 3 * n_angles), 0);
+    for (int a = 0; a < n_angles; ++a) {
+        const int t = anglelist[4 * a] - 1, i = anglelist[4 * a + 1], j = anglelist[4 * a + 2],
+                  k = anglelist[4 * a + 3];
+        const int mem[3] = {i, j, k};
+        for (int role = 0; role < 3; ++role) {
+            int *e = &ent[4 * (size_t)fillp[mem[role] - 1]++];
+            e[0] = t | (role << 8);
+            e[1] = role == 0 ? j : i;
+            e[2] = role == 2 ? j : k;
+            e[3] = 0;
+        }
+    }
+    for (int t = 0; t < n_angle_types; ++t) {
+        h.angle_k[t] = angle_coeff[2 * t];
+        h.angle_th0[t] = angle_coeff[2 * t + 1] * (M_PI / 180.0);
+    }
+    int rc;
+    if ((rc = dalloc(ctx, s, &h.angle_ent, ent.size()))) return rc;
+    HIPCHK(hipMemcpyAsync(h.angle_start, start.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.angle_ent, ent.data(), sizeof(int) * ent.size(), hipMemcpyHostToDevice, ctx->stream));
+    h.nangles = n_angles;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return push_desc(ctx, sid);
+}
+
 // ---- reference neighbour list ---------------------------------------------------
 
 int gdpd_neigh_build(gdpd_ctx *ctx, int sid)
@@ -1131,6 +1185,20 @@ int gdpd_bond_compute(gdpd_ctx *ctx, int sid, uint32_t step, int log)
     return GDPD_OK;
 }
 
+int gdpd_angle_compute(gdpd_ctx *ctx, int sid, uint32_t step, int log)
+{
+    (void)step;
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    if (s->h.nangles == 0) return GDPD_OK;
+    launch_angle_hook(ctx->d_reps, s->tab.beads, s->tab.n_beads, log, ctx->stream);
+    if ((rc = check_launch(ctx))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
 static int nve_hook(gdpd_ctx *ctx, int sid, int initial)
 {
     Sys *s = get_sys(ctx, sid);
@@ -1175,6 +1243,7 @@ int gdpd_thermo(gdpd_ctx *ctx, int sid, double *out)
     out[3] = (out[4] + out[7] + out[9]) / 3.0;
     out[10] = tot[0];
     out[11] = -1.0;
+    out[12] = tot[9];
     return GDPD_OK;
 }
 
@@ -1243,12 +1312,13 @@ int gdpd_thermo_rows(gdpd_ctx *ctx, int sid, double *out, int max_rows, int *n_r
     const int cap = s->h.thermo_cap;
     const int have = std::min(st.log_rows, cap);
     const int first = st.log_rows - have;
-    std::vector<double> rows((size_t)cap * 12);
+    std::vector<double> rows((size_t)cap * kThermoRowLen);
     HIPCHK(hipMemcpy(rows.data(), s->h.thermo_rows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
     const int n = std::min(have, max_rows);
     for (int k = 0; k < n && out; ++k) {
         const int r = (first + k) % cap;
-        memcpy(out + (size_t)k * GDPD_THERMO_LEN, rows.data() + (size_t)r * 12, sizeof(double) * 12);
+        memcpy(out + (size_t)k * GDPD_THERMO_LEN, rows.data() + (size_t)r * kThermoRowLen,
+               sizeof(double) * kThermoRowLen);
     }
     *n_rows = n;
     return GDPD_OK;

@@ -8,8 +8,10 @@ namespace gdpd {
 
 constexpr int kMaxTypes = 12;      // the reference's 256-entry constant tables cap DPD at 12 types
 constexpr int kMaxBondTypes = 16;
+constexpr int kMaxAngleTypes = 16;
 constexpr int kBlock = 256;        // threads per workgroup for every kernel
-constexpr int kThermoParts = 10;   // ke, pair pe, bond pe, p[6], unused
+constexpr int kThermoParts = 10;   // ke, pair pe, bond pe, p[6], angle pe
+constexpr int kThermoRowLen = 13;  // one device thermo record (GDPD_THERMO_LEN)
 constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // device flag bits (RepState::flags)
@@ -58,6 +60,7 @@ struct RepDev {
     double dt;
     double mass[kMaxTypes], minv[kMaxTypes];
     double bond_k[kMaxBondTypes], bond_r0[kMaxBondTypes];
+    double angle_k[kMaxAngleTypes], angle_th0[kMaxAngleTypes];  // theta0 in radians
 
     double *x[2], *v[2];       // SoA, 3N each: [X..., Y..., Z...]
     int *id;                   // 1-based ids
@@ -75,10 +78,16 @@ struct RepDev {
     int nbonds;
     int *bond_start, *bond_partner;  // CSR by id-1: partner ids (1-based)
     unsigned char *bond_type;        // per CSR entry, 0-based
+    int nangles;
+    // angles a bead belongs to, CSR by id-1; 4 ints per entry:
+    // type-1 | role << 8 (0: end i, 1: vertex j, 2: end k), then the other two
+    // member ids in i, j, k order
+    int *angle_start, *angle_ent;
+    double *ape;                     // N: per-bead third of its angles' energy (log steps)
     const void *coef;          // PairCoef<real>[ntypes * ntypes]
     double *thermo_part;       // [nbrick][kThermoParts]: pair PE and virial of log steps
     double *thermo_bead;       // [ceil(N/256)][kThermoParts]: KE, bond terms, m v v
-    double *thermo_rows;       // [thermo_cap][12]
+    double *thermo_rows;       // [thermo_cap][kThermoRowLen]
     int thermo_cap;
     RepState *state;
     // scratch for re-sorting

@@ -37,6 +37,7 @@ void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int fli
 // hook path
 void launch_nve(const RepDev *reps, const WorkItem *items, int nitems, int initial, hipStream_t s);
 void launch_bond_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s);
+void launch_angle_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s);
 void launch_thermo_arrays(const RepDev *reps, const WorkItem *items, int nitems, double *partials,
                           hipStream_t s);
 // reference neighbour list (single replica)

@@ -984,12 +984,65 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         R.force[slot] = (double)L.facc[3 * o + 0] * FixScale<real>::inv;
         R.force[N + slot] = (double)L.facc[3 * o + 1] * FixScale<real>::inv;
         R.force[2 * N + slot] = (double)L.facc[3 * o + 2] * FixScale<real>::inv;
-        if (log && bp.perbead) R.bpe[slot] = 0.0;
+        if (log && bp.perbead) {
+            R.bpe[slot] = 0.0;
+            R.ape[slot] = 0.0;
+        }
     }
     if (log && !bp.perbead) {
         long long *red = (long long *)L.red, *out = red + 48;
         block_sum_i64<kThermoParts>(acc, red, out);
         if (t < kThermoParts) ((long long *)R.thermo_part)[(size_t)it.local * kThermoParts + t] = out[t];
+    }
+}
+
+// One harmonic angle's force on bead s (force/angle_harmonic.cu:168-330: cos
+// clamped to [-1, 1], sin floored at 0.001, a = -2k (theta - theta0) / sin),
+// from the bead's CSR entry {type | role << 8, other ids}.  Adds the force to
+// f; with log, adds a third of the angle energy to pe and a third of its
+// virial to w.  Every member evaluates the same angle with the same
+// arithmetic, so the three forces sum to zero to rounding.
+__device__ __forceinline__ void angle_on_bead(const RepDev &R, const double *X, int s, const int *ent, double (&f)[3],
+                                              bool log, double &pe, double (&w)[6])
+{
+    const int N = R.N;
+    const int tr = ent[0], role = tr >> 8, at = tr & 255;
+    const int sa = R.slot_of_id[ent[1] - 1], sb = R.slot_of_id[ent[2] - 1];
+    const int si = role == 0 ? s : sa;
+    const int sj = role == 1 ? s : (role == 0 ? sa : sb);
+    const int sk = role == 2 ? s : sb;
+    double d1[3], d2[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        d1[k] = min_image(X[k * N + si] - X[k * N + sj], R.L[k], R.hL[k]);
+        d2[k] = min_image(X[k * N + sk] - X[k * N + sj], R.L[k], R.hL[k]);
+    }
+    const double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+    const double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+    const double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
+    double c = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2);
+    c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+    double sn = sqrt(1.0 - c * c);
+    sn = sn < 0.001 ? 0.001 : sn;
+    const double kk = R.angle_k[at];
+    const double dth = acos(c) - R.angle_th0[at];
+    const double pre = -2.0 * kk * dth / sn;
+    const double a11 = pre * c / rsq1, a12 = -pre / (r1 * r2), a22 = pre * c / rsq2;
+    double f1[3], f3[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f1[k] = a11 * d1[k] + a12 * d2[k];
+        f3[k] = a22 * d2[k] + a12 * d1[k];
+        f[k] += role == 0 ? f1[k] : (role == 2 ? f3[k] : -(f1[k] + f3[k]));
+    }
+    if (log) {
+        pe += (kk * dth * dth) / 3.0;
+        w[0] += (d1[0] * f1[0] + d2[0] * f3[0]) / 3.0;
+        w[1] += (d1[0] * f1[1] + d2[0] * f3[1]) / 3.0;
+        w[2] += (d1[0] * f1[2] + d2[0] * f3[2]) / 3.0;
+        w[3] += (d1[1] * f1[1] + d2[1] * f3[1]) / 3.0;
+        w[4] += (d1[1] * f1[2] + d2[1] * f3[2]) / 3.0;
+        w[5] += (d1[2] * f1[2] + d2[2] * f3[2]) / 3.0;
     }
 }
 
@@ -1050,11 +1103,22 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
                     bw[5] += 0.5 * ddz * bfz;
                 }
             }
-            if (mode & kModeStoreForce) {
-                R.force[s] = fx;
-                R.force[N + s] = fy;
-                R.force[2 * N + s] = fz;
-            }
+        }
+        double ape = 0.0;
+        if (R.nangles > 0) {
+            double fa[3] = {fx, fy, fz};
+            const int idi = R.id[s];
+            const int a1 = R.angle_start[idi];
+            for (int aa = R.angle_start[idi - 1]; aa < a1; ++aa)
+                angle_on_bead(R, X, s, R.angle_ent + 4 * (size_t)aa, fa, log, ape, bw);
+            fx = fa[0];
+            fy = fa[1];
+            fz = fa[2];
+        }
+        if ((R.nbonds > 0 || R.nangles > 0) && (mode & kModeStoreForce)) {
+            R.force[s] = fx;
+            R.force[N + s] = fy;
+            R.force[2 * N + s] = fz;
         }
         const int tt = R.type[s] - 1;
         const double h = (0.5 * R.dt) * R.minv[tt];
@@ -1070,6 +1134,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             const double m = R.mass[tt];
             acc[0] = 0.5 * m * (vxg * vxg + vyg * vyg + vzg * vzg);
             acc[2] = bpe;
+            acc[9] = ape;
             acc[3] = m * vxg * vxg + bw[0];
             acc[4] = m * vxg * vyg + bw[1];
             acc[5] = m * vxg * vzg + bw[2];
@@ -1198,7 +1263,7 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
         if (threadIdx.x == 0) {
             const double vol = R.L[0] * R.L[1] * R.L[2];
             const int row = R.state->log_rows % R.thermo_cap;
-            double *o = R.thermo_rows + (size_t)row * 12;
+            double *o = R.thermo_rows + (size_t)row * kThermoRowLen;
             const double ndof = 3.0 * R.N - 3.0;
             o[0] = (2.0 * tot[0]) / (ndof * 1.0);
             o[1] = tot[1];
@@ -1207,6 +1272,7 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
             o[3] = (o[4] + o[7] + o[9]) / 3.0;
             o[10] = tot[0];
             o[11] = (double)step;
+            o[12] = tot[9];
             R.state->log_rows += 1;
         }
     }
@@ -1294,6 +1360,30 @@ __global__ __launch_bounds__(kBlock) void k_bond_hook(const RepDev *__restrict__
     }
 }
 
+// Angle::compute_force on the hook path: each bead adds its angles to the
+// force in the order the fused integration kernel uses
+__global__ __launch_bounds__(kBlock) void k_angle_hook(const RepDev *__restrict__ reps,
+                                                       const WorkItem *__restrict__ items,
+                                                       int nitems, int log)
+{
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    if (s >= R.N) return;
+    const int N = R.N;
+    const double *X = R.x[R.state->cur];
+    const int id = R.id[s];
+    double f[3] = {R.force[s], R.force[N + s], R.force[2 * N + s]};
+    double w[6] = {0, 0, 0, 0, 0, 0}, ape = 0.0;
+    const int a1 = R.angle_start[id];
+    for (int aa = R.angle_start[id - 1]; aa < a1; ++aa) angle_on_bead(R, X, s, R.angle_ent + 4 * (size_t)aa, f, log != 0, ape, w);
+    for (int k = 0; k < 3; ++k) R.force[k * N + s] = f[k];
+    if (log) {
+        R.ape[s] = ape;
+        for (int k = 0; k < 6; ++k) R.vir[k * N + s] += w[k];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_thermo_arrays(const RepDev *__restrict__ reps,
                                                           const WorkItem *__restrict__ items,
                                                           double *__restrict__ partials)
@@ -1314,6 +1404,7 @@ __global__ __launch_bounds__(kBlock) void k_thermo_arrays(const RepDev *__restri
         acc[0] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
         acc[1] = R.pe[s];
         acc[2] = R.bpe[s];
+        acc[9] = R.ape[s];
         acc[3] = m * vx * vx + R.vir[s];
         acc[4] = m * vx * vy + R.vir[N + s];
         acc[5] = m * vx * vz + R.vir[2 * N + s];
@@ -1554,6 +1645,11 @@ void launch_nve(const RepDev *reps, const WorkItem *items, int nitems, int initi
 void launch_bond_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s)
 {
     if (nitems) hipLaunchKernelGGL(k_bond_hook, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, log);
+}
+
+void launch_angle_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_angle_hook, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, log);
 }
 void launch_thermo_arrays(const RepDev *reps, const WorkItem *items, int nitems, double *partials,
                           hipStream_t s)

@@ -152,6 +152,12 @@ class System:
         self.ctx.check(self.lib.gdpd_set_bonds(self.ctx.h, self.sid, len(b), int(n_bond_types),
                                                _ptr(b, C.c_int), _ptr(c, C.c_double)))
 
+    def set_angles(self, anglelist, n_angle_types, angle_coeff):
+        a = np.ascontiguousarray(anglelist, dtype=np.int32).reshape(-1, 4)
+        c = np.ascontiguousarray(angle_coeff, dtype=np.float64)
+        self.ctx.check(self.lib.gdpd_set_angles(self.ctx.h, self.sid, len(a), int(n_angle_types),
+                                                _ptr(a, C.c_int), _ptr(c, C.c_double)))
+
     # ---- hooks ----
     def neigh_build(self):
         self.ctx.check(self.lib.gdpd_neigh_build(self.ctx.h, self.sid))
@@ -173,6 +179,9 @@ class System:
 
     def bond_compute(self, step, log=True):
         self.ctx.check(self.lib.gdpd_bond_compute(self.ctx.h, self.sid, int(step), 1 if log else 0))
+
+    def angle_compute(self, step, log=True):
+        self.ctx.check(self.lib.gdpd_angle_compute(self.ctx.h, self.sid, int(step), 1 if log else 0))
 
     def nve_initial(self):
         self.ctx.check(self.lib.gdpd_nve_initial(self.ctx.h, self.sid))

@@ -38,7 +38,7 @@ extern "C" {
 #define GDPD_STYLE_DPD 1
 #define GDPD_STYLE_MDPD 2
 
-/* thermo record layout (compute/thermo.cu:122-214 columns, plus KE and step) */
+/* thermo record layout (compute/thermo.cu:122-214 columns, plus KE, step and angle PE) */
 #define GDPD_THERMO_T 0
 #define GDPD_THERMO_PAIR_PE 1
 #define GDPD_THERMO_BOND_PE 2
@@ -46,7 +46,8 @@ extern "C" {
 #define GDPD_THERMO_PXX 4 /* 4..9: Pxx Pxy Pxz Pyy Pyz Pzz */
 #define GDPD_THERMO_KE 10
 #define GDPD_THERMO_STEP 11
-#define GDPD_THERMO_LEN 12
+#define GDPD_THERMO_ANGLE_PE 12
+#define GDPD_THERMO_LEN 13
 
 typedef struct gdpd_ctx gdpd_ctx;
 
@@ -105,6 +106,12 @@ int gdpd_set_pair_mdpd(gdpd_ctx *ctx, int sys, double temp, double rc, double rd
 int gdpd_set_bonds(gdpd_ctx *ctx, int sys, int n_bonds, int n_bond_types, const int *bondlist,
                    const double *bond_coeff);
 
+/* replaces Angle_harmonic::isEnabled + angles_mem_copy_2_gpu (force/angle_harmonic.cu:19-151,
+   system/system.cu:180-215): anglelist 4 ints per angle {type, id_i, id_j, id_k}, j the vertex;
+   angle_coeff 2 doubles per type {k, theta0 in degrees}, E = k (theta - theta0)^2. */
+int gdpd_set_angles(gdpd_ctx *ctx, int sys, int n_angles, int n_angle_types, const int *anglelist,
+                    const double *angle_coeff);
+
 /* ---- plugin-hook path (one reference hook per call) ----------------------- */
 /* Neigh_list::build (force/neigh_list.cu:397-479): half Verlet list, bins of
    global_cut + skin, r^2 <= (global_cut+skin)^2, i < j by index. */
@@ -123,6 +130,8 @@ int gdpd_neigh_download(gdpd_ctx *ctx, int sys, int *n_neigh, int *neigh, int n_
 int gdpd_pair_compute(gdpd_ctx *ctx, int sys, uint32_t step, int log);
 /* Bond::compute_force (force/bond_harmonic.cu:355-402): adds bond forces. */
 int gdpd_bond_compute(gdpd_ctx *ctx, int sys, uint32_t step, int log);
+/* Angle::compute_force (force/angle_harmonic.cu:459-...; kernels :168-458): adds angle forces. */
+int gdpd_angle_compute(gdpd_ctx *ctx, int sys, uint32_t step, int log);
 /* nve::initial_integrate / final_integrate (fix/fix_nve.cu:93-118, 156-177) */
 int gdpd_nve_initial(gdpd_ctx *ctx, int sys);
 int gdpd_nve_final(gdpd_ctx *ctx, int sys);

@@ -17,6 +17,8 @@
  *                     2-bead force closed form; third law.
  *   orc_bond_force    closed form F = -2k(r-r0) r_hat, E = k(r-r0)^2; a 2-bead
  *                     bonded NVE run conserves energy.
+ *   orc_angle_force   F = -grad of E = k (theta - theta0)^2 by central differences,
+ *                     zero net force and torque, zero force at theta0.
  *   orc_nve_*         free particle x(t) = wrap(x0 + v t), unwrapped drift.
  *   orc_thermo        2-bead closed forms; T = 2KE/(3N-3).
  *   orc_run           step ordering against a hand-sequenced run of the pieces.
@@ -475,6 +477,57 @@ static inline double wrap_coord(double x, double lo, double hi, double len)
 {
     while (x >= hi || x < lo) x -= len * (double)((x >= hi) - (x < lo));
     return x;
+}
+
+/* Harmonic angles, force/angle_harmonic.cu:168-330: cos(theta) clamped to
+   [-1, 1], sin(theta) floored at 0.001, a = -2 k (theta - theta0) / sin(theta). */
+void orc_angle_force(int n_angles, const int *anglelist, const double *angle_coeff, const double *pos,
+                     const orc_box *box, double *force, double *angle_pe, double *virial)
+{
+    double L[3], hL[3];
+    for (int d = 0; d < 3; ++d) {
+        L[d] = box->hi[d] - box->lo[d];
+        hL[d] = 0.5 * L[d];
+    }
+    for (int a = 0; a < n_angles; ++a) {
+        const int at = anglelist[4 * a + 0] - 1;
+        const int i = anglelist[4 * a + 1] - 1, j = anglelist[4 * a + 2] - 1, k = anglelist[4 * a + 3] - 1;
+        const double kk = angle_coeff[2 * at + 0];
+        const double theta0 = angle_coeff[2 * at + 1] * (M_PI / 180.0);
+        double d1[3], d2[3];
+        for (int d = 0; d < 3; ++d) {
+            d1[d] = min_image(pos[3 * i + d] - pos[3 * j + d], L[d], hL[d]);
+            d2[d] = min_image(pos[3 * k + d] - pos[3 * j + d], L[d], hL[d]);
+        }
+        const double rsq1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+        const double rsq2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+        const double r1 = sqrt(rsq1), r2 = sqrt(rsq2);
+        double c = (d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2]) / (r1 * r2);
+        if (c > 1.0) c = 1.0;
+        if (c < -1.0) c = -1.0;
+        double sn = sqrt(1.0 - c * c);
+        if (sn < 0.001) sn = 0.001;
+        const double dth = acos(c) - theta0;
+        const double pre = -2.0 * kk * dth / sn;
+        const double a11 = pre * c / rsq1, a12 = -pre / (r1 * r2), a22 = pre * c / rsq2;
+        double f1[3], f3[3];
+        for (int d = 0; d < 3; ++d) {
+            f1[d] = a11 * d1[d] + a12 * d2[d];
+            f3[d] = a22 * d2[d] + a12 * d1[d];
+            force[3 * i + d] += f1[d];
+            force[3 * k + d] += f3[d];
+            force[3 * j + d] -= f1[d] + f3[d];
+        }
+        if (angle_pe) angle_pe[a] = kk * dth * dth;
+        if (virial) {
+            const double w[6] = {(d1[0] * f1[0] + d2[0] * f3[0]) / 3.0, (d1[0] * f1[1] + d2[0] * f3[1]) / 3.0,
+                                 (d1[0] * f1[2] + d2[0] * f3[2]) / 3.0, (d1[1] * f1[1] + d2[1] * f3[1]) / 3.0,
+                                 (d1[1] * f1[2] + d2[1] * f3[2]) / 3.0, (d1[2] * f1[2] + d2[2] * f3[2]) / 3.0};
+            const int who[3] = {i, j, k};
+            for (int m = 0; m < 3; ++m)
+                for (int c6 = 0; c6 < 6; ++c6) virial[6 * who[m] + c6] += w[c6];
+        }
+    }
 }
 
 void orc_nve_initial(int N, double dt, const int *type, const double *masses, const double *force,

@@ -76,6 +76,14 @@ void orc_mdpd_force(int N, int ntypes, const int *id, const int *type, const dou
 void orc_bond_force(int n_bonds, const int *bondlist, const double *bond_coeff, const double *pos,
                     const orc_box *box, double *force, double *bond_pe, double *virial);
 
+/* ---- harmonic angles (force/angle_harmonic.cu:168-330) ----
+   anglelist: 4 ints per angle {type, id_i, id_j, id_k} (1-based, j the vertex);
+   angle_coeff: {k, theta0 in degrees} per type; E = k (theta - theta0)^2.
+   Accumulates into force (and angle_pe[a], virial when non-NULL; each of the
+   three beads gets a third of the angle's virial). */
+void orc_angle_force(int n_angles, const int *anglelist, const double *angle_coeff, const double *pos,
+                     const orc_box *box, double *force, double *angle_pe, double *virial);
+
 /* ---- nve (fix/fix_nve.cu:30-89, 122-152); masses indexed by type-1 ---- */
 void orc_nve_initial(int N, double dt, const int *type, const double *masses, const double *force,
                      const orc_box *box, double *pos, double *uwpos, double *vel);

@@ -67,6 +67,7 @@ def lib():
                                      C.c_int, _PD, C.c_double, C.c_double, C.c_uint32, C.c_uint32, _PD, _PD,
                                      _PD]
         L.orc_bond_force.argtypes = [C.c_int, _PI, _PD, _PD, C.POINTER(Box), _PD, _PD, _PD]
+        L.orc_angle_force.argtypes = [C.c_int, _PI, _PD, _PD, C.POINTER(Box), _PD, _PD, _PD]
         L.orc_nve_initial.argtypes = [C.c_int, C.c_double, _PI, _PD, _PD, C.POINTER(Box), _PD, _PD, _PD]
         L.orc_nve_final.argtypes = [C.c_int, C.c_double, _PI, _PD, _PD, _PD]
         L.orc_thermo.argtypes = [C.c_int, _PI, _PD, _PD, _PD, _PD, C.POINTER(Box), C.c_int, _PD, _PD]
@@ -167,7 +168,25 @@ def forces(cfg, pos=None, vel=None, step=0, with_list=None):
         L.orc_bond_force(len(bl), p(bl, C.c_int), p(np.ascontiguousarray(cfg["bond_coeff"], dtype=np.float64)),
                          p(pos), C.byref(b), p(f), p(bpe), p(vir))
         bpe_total = float(bpe.sum())
+    if "angles" in cfg:
+        fa, _, va = angle_force(cfg["angles"], cfg["angle_coeff"], pos, cfg["lo"], cfg["hi"])
+        f += fa
+        vir += va
     return f, pe, bpe_total, vir, rho
+
+
+def angle_force(anglelist, angle_coeff, pos, lo, hi):
+    """Harmonic angle force, per-angle PE and virial at positions pos (N,3),
+    rows indexed by id - 1.  Returns (force (N,3), angle_pe (n_angles,), virial (N,6))."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    N = len(pos)
+    al = np.ascontiguousarray(anglelist, dtype=np.int32)
+    f, vir = np.zeros((N, 3)), np.zeros((N, 6))
+    ape = np.zeros(len(al))
+    b = box(lo, hi)
+    lib().orc_angle_force(len(al), p(al, C.c_int), p(np.ascontiguousarray(angle_coeff, dtype=np.float64)),
+                          p(pos), C.byref(b), p(f), p(ape), p(vir))
+    return f, ape, vir
 
 
 def make_orc_system(cfg, log_every=100, check_every=1):

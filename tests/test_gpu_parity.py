@@ -162,3 +162,32 @@ def test_mdpd_brick_shapes_match_oracle(ctx, brick):
     assert rel(f, f_ref) < F64_TOL
     assert rel(pe, pe_ref) < F64_TOL
     s.destroy()
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)])
+def test_angles_match_oracle(ctx, prec, tol):
+    """Harmonic angles (force/angle_harmonic.cu:168-330) on a semi-flexible melt:
+    pair + bond + angle force and virial, and the angle energy of the hook
+    thermo and of the fused run's step-0 record."""
+    from oracle import pyoracle as O
+    cfg = configs.config_angles(n_chains=700, box=15.0)
+    print(f"case: angle melt prec={prec}", flush=True)
+    s = g.system_from_config(ctx, cfg, precision=prec)
+    s.pair_compute(step=2, log=True)
+    s.bond_compute(step=2, log=True)
+    s.angle_compute(step=2, log=True)
+    f = s.download()["force"]
+    _, vir, _ = s.download_pe_virial()
+    th = s.thermo()
+    s.destroy()
+    f_ref, _, _, vir_ref, _ = pyoracle.forces(cfg, step=2)
+    order = np.argsort(cfg["id"])
+    _, ape, _ = O.angle_force(cfg["angles"], cfg["angle_coeff"], np.asarray(cfg["pos"])[order], cfg["lo"], cfg["hi"])
+    assert rel(f, f_ref) < tol
+    assert rel(vir, vir_ref) < tol
+    assert th[12] == pytest.approx(ape.sum(), rel=max(tol, 1e-12))
+    r = g.system_from_config(ctx, cfg, precision=prec, log_every=1)
+    r.run(0)
+    row = r.thermo_rows()[0]
+    r.destroy()
+    assert row[12] == pytest.approx(ape.sum(), rel=max(tol, 1e-12))

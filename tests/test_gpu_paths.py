@@ -122,3 +122,26 @@ def test_bonded_hook_path_equals_fused_path(ctx):
     assert np.array_equal(a.download()["force"], b.download()["force"])
     a.destroy()
     b.destroy()
+
+
+def test_angle_hook_path_equals_fused_path(ctx):
+    cfg = configs.config_angles(n_chains=600, box=14.0)
+    n = 8
+    a = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=4)
+    a.run(n)
+    b = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=4)
+    b.pair_compute(0, log=True)
+    b.bond_compute(0, log=True)
+    b.angle_compute(0, log=True)
+    for step in range(1, n + 1):
+        b.nve_initial()
+        b.pair_compute(step, log=False)
+        b.bond_compute(step, log=False)
+        b.angle_compute(step, log=False)
+        b.nve_final()
+    pa, va = state(a)
+    pb, vb = state(b)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    assert np.array_equal(a.download()["force"], b.download()["force"])
+    a.destroy()
+    b.destroy()

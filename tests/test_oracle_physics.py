@@ -297,3 +297,56 @@ def test_pressure_mean_field_at_uniform_start_and_thermostat():
     assert rows[20:, 0].mean() == pytest.approx(1.0, abs=0.06)
     # the list rebuilds almost every step at dt=0.04, skin=0.3
     assert builds > 250
+
+
+# ---- harmonic angles (force/angle_harmonic.cu:168-330) ---------------------------
+
+def _angle_energy(al, coeff, pos, lo, hi):
+    return float(O.angle_force(al, coeff, pos, lo, hi)[1].sum())
+
+
+def test_angle_closed_form_right_angle():
+    # i on x, vertex at the origin, k on y: theta = 90 deg; theta0 = 60 deg
+    lo, hi = [-10.0] * 3, [10.0] * 3
+    pos = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    k = 3.0
+    f, ape, vir = O.angle_force([[1, 1, 2, 3]], [k, 60.0], pos, lo, hi)
+    dth = np.pi / 2 - np.pi / 3
+    assert ape[0] == pytest.approx(k * dth * dth, rel=1e-14)
+    # -dE/dr pulls i toward k and k toward i
+    assert np.allclose(f[0], [0.0, 2 * k * dth, 0.0], atol=1e-13)
+    assert np.allclose(f[2], [2 * k * dth, 0.0, 0.0], atol=1e-13)
+    assert np.allclose(f[1], -(f[0] + f[2]), atol=1e-13)
+
+
+def test_angle_force_is_gradient_of_angle_energy():
+    rng = np.random.default_rng(11)
+    lo, hi = [0.0] * 3, [8.0] * 3
+    n = 12
+    pos = np.cumsum(rng.normal(0.0, 0.6, (n, 3)), axis=0) % 8.0
+    al = np.array([[1 + (a % 2), a + 1, a + 2, a + 3] for a in range(n - 2)], dtype=np.int32)
+    coeff = [4.0, 120.0, 2.5, 75.0]
+    f, _, _ = O.angle_force(al, coeff, pos, lo, hi)
+    h = 1e-6
+    for i in range(n):
+        for d in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, d] += h
+            pm[i, d] -= h
+            g = (_angle_energy(al, coeff, pp, lo, hi) - _angle_energy(al, coeff, pm, lo, hi)) / (2 * h)
+            assert f[i, d] == pytest.approx(-g, rel=1e-5, abs=1e-6)
+
+
+def test_angle_forces_have_no_net_force_or_torque_and_vanish_at_theta0():
+    rng = np.random.default_rng(12)
+    lo, hi = [-50.0] * 3, [50.0] * 3
+    pos = rng.normal(0.0, 1.0, (3, 3))
+    f, _, vir = O.angle_force([[1, 1, 2, 3]], [5.0, 100.0], pos, lo, hi)
+    assert np.allclose(f.sum(axis=0), 0.0, atol=1e-12)
+    assert np.allclose(np.cross(pos, f).sum(axis=0), 0.0, atol=1e-11)
+    # the three thirds of the virial are equal
+    assert np.allclose(vir[0], vir[1]) and np.allclose(vir[1], vir[2])
+    d1, d2 = pos[0] - pos[1], pos[2] - pos[1]
+    th = np.degrees(np.arccos(d1 @ d2 / np.linalg.norm(d1) / np.linalg.norm(d2)))
+    f0, ape0, _ = O.angle_force([[1, 1, 2, 3]], [5.0, th], pos, lo, hi)
+    assert np.allclose(f0, 0.0, atol=1e-9) and ape0[0] < 1e-18

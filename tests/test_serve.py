@@ -28,7 +28,7 @@ class FakeSys:
         return {"pos": np.zeros((1, 3)), "vel": np.zeros((1, 3))}
 
     def thermo_rows(self):
-        return np.zeros((0, 12))
+        return np.zeros((0, 13))
 
 
 class FakeCtx:
