@@ -98,7 +98,8 @@ void read_input(System &s, const std::string &text)
             if (a.size() != 1) illegal(l, "illegal atom_style command", fmt);
             if (a[0] == "atomic") s.atom_style = 0;
             else if (a[0] == "bond") s.atom_style = 1;
-            else illegal(l, "illegal atom_style command (atomic or bond)", fmt);
+            else if (a[0] == "angle") s.atom_style = 2;
+            else illegal(l, "illegal atom_style command", fmt);
         } else if (key == "neighbor") {
             const char *fmt = "neighbor global_cut skin max_rho check_frequency";
             if (a.size() != 4) illegal(l, "illegal neighbor command", fmt);
@@ -135,10 +136,10 @@ void read_input(System &s, const std::string &text)
             s.run_p.dumpforce_flag = a[3] == "force_yes";
             s.run_p.dumpvel_flag = a[4] == "vel_yes";
             s.run_p.write_dump_flag = true;
-        } else if (key == "angle_style" || key == "angle_coeff" || key == "compute") {
+        } else if (key == "compute") {
             throw Error("\n   Error: " + key + " is not supported by this build\n   Line: " + l);
         } else if (key != "pair_style" && key != "pair_coeff" && key != "bond_style" && key != "bond_coeff" &&
-                   key != "group" && key != "fix") {
+                   key != "angle_style" && key != "angle_coeff" && key != "group" && key != "fix") {
             // main/read_in.cu:295-305
             throw Error("unknown command " + key);
         }
@@ -200,7 +201,7 @@ void read_data(System &s, const std::string &path)
     std::vector<std::string> lines;
     for (std::string l; std::getline(f, l);) lines.push_back(l);
     // line 0 is the title; header counts and box lines follow until a section
-    long n_atoms = -1, n_bonds = 0, n_at = -1, n_bt = 0;
+    long n_atoms = -1, n_bonds = 0, n_at = -1, n_bt = 0, n_angles = 0, n_ant = 0;
     bool box[3] = {false, false, false};
     size_t i = 1;
     for (; i < lines.size(); ++i) {
@@ -212,10 +213,10 @@ void read_data(System &s, const std::string &path)
             if (!(is >> n_at)) throw Error("Error reading number of atom types from line: " + lines[i]);
         } else if (ends_with(t, "bond types")) {
             if (!(is >> n_bt)) throw Error("Error reading number of bond types from line: " + lines[i]);
-        } else if (ends_with(t, "angle types") || ends_with(t, "angles")) {
-            long n = 0;
-            is >> n;
-            if (n > 0) throw Error("data file has angles: atom_style angle is not supported by this build");
+        } else if (ends_with(t, "angle types")) {
+            if (!(is >> n_ant)) throw Error("Error reading number of angle types from line: " + lines[i]);
+        } else if (ends_with(t, "angles")) {
+            if (!(is >> n_angles)) throw Error("Error reading number of angles from line: " + lines[i]);
         } else if (ends_with(t, "atoms")) {
             if (!(is >> n_atoms)) throw Error("Error reading number of atoms from line: " + lines[i]);
         } else if (ends_with(t, "bonds")) {
@@ -239,10 +240,14 @@ void read_data(System &s, const std::string &path)
     if (s.box.xlo >= s.box.xhi || s.box.ylo >= s.box.yhi || s.box.zlo >= s.box.zhi)
         throw Error("Invalid box dimensions.");
     if (s.atom_style == 0 && n_bonds > 0) throw Error("data file has bonds but atom_style is atomic");
+    if (s.atom_style != 2 && n_angles > 0) throw Error("data file has angles but atom_style is not angle");
+    const bool molecular = s.atom_style >= 1;
     s.n_atoms = (int)n_atoms;
     s.n_atomtypes = (int)n_at;
-    s.n_bonds = s.atom_style == 1 ? (int)n_bonds : 0;
-    s.n_bondtypes = s.atom_style == 1 ? (int)n_bt : 0;
+    s.n_bonds = molecular ? (int)n_bonds : 0;
+    s.n_bondtypes = molecular ? (int)n_bt : 0;
+    s.n_angles = s.atom_style == 2 ? (int)n_angles : 0;
+    s.n_angletypes = s.atom_style == 2 ? (int)n_ant : 0;
     s.box.lx = s.box.xhi - s.box.xlo;
     s.box.ly = s.box.yhi - s.box.ylo;
     s.box.lz = s.box.zhi - s.box.zlo;
@@ -255,10 +260,11 @@ void read_data(System &s, const std::string &path)
     s.h_type.assign(N, 0);
     s.h_pos.assign(3 * N, 0.0);
     s.h_vel.assign(3 * N, 0.0);
-    s.h_mol_id.assign(s.atom_style == 1 ? N : 0, 0);
+    s.h_mol_id.assign(molecular ? N : 0, 0);
     s.h_bondlist.assign(3 * (size_t)s.n_bonds, -1);
+    s.h_anglelist.assign(4 * (size_t)s.n_angles, -1);
     s.has_velocities = false;
-    bool got_atoms = false, got_masses = false, got_bonds = false;
+    bool got_atoms = false, got_masses = false, got_bonds = false, got_angles = false;
     const double lo[3] = {s.box.xlo, s.box.ylo, s.box.zlo}, hi[3] = {s.box.xhi, s.box.yhi, s.box.zhi};
     while (i < lines.size()) {
         const std::string raw = lines[i];
@@ -291,8 +297,8 @@ void read_data(System &s, const std::string &path)
                 std::istringstream is(r);
                 long id, mol = 0, ty;
                 double x[3];
-                bool ok = s.atom_style == 1 ? (bool)(is >> id >> mol >> ty >> x[0] >> x[1] >> x[2])
-                                            : (bool)(is >> id >> ty >> x[0] >> x[1] >> x[2]);
+                bool ok = molecular ? (bool)(is >> id >> mol >> ty >> x[0] >> x[1] >> x[2])
+                                    : (bool)(is >> id >> ty >> x[0] >> x[1] >> x[2]);
                 if (!ok) throw Error("data file: bad Atoms row: " + r);
                 if (id == 0) throw Error("Atom ID cannot be 0.");
                 if (id < 1 || id > n_atoms) throw Error("data file: atom id out of range: " + r);
@@ -301,7 +307,7 @@ void read_data(System &s, const std::string &path)
                 const size_t k = (size_t)id - 1;
                 s.h_id[k] = (int)id;
                 s.h_type[k] = (int)ty;
-                if (s.atom_style == 1) s.h_mol_id[k] = (int)mol;
+                if (molecular) s.h_mol_id[k] = (int)mol;
                 for (int d = 0; d < 3; ++d) {
                     // Read_data::adjust_coordinate (system/read_data.cu:191-197)
                     double v = x[d];
@@ -321,7 +327,7 @@ void read_data(System &s, const std::string &path)
             }
             s.has_velocities = true;
         } else if (t == "Bonds") {
-            if (s.atom_style != 1) throw Error("data file has a Bonds section but atom_style is atomic");
+            if (!molecular) throw Error("data file has a Bonds section but atom_style is atomic");
             for (const std::string &r : section_rows(lines, i, n_bonds, t)) {
                 std::istringstream is(r);
                 long bid, bt, a, b;
@@ -337,7 +343,22 @@ void read_data(System &s, const std::string &path)
             }
             got_bonds = true;
         } else if (t == "Angles") {
-            throw Error("data file has angles: atom_style angle is not supported by this build");
+            if (s.atom_style != 2) throw Error("data file has an Angles section but atom_style is not angle");
+            for (const std::string &r : section_rows(lines, i, n_angles, t)) {
+                std::istringstream is(r);
+                long aid, at, a, b, c;
+                if (!(is >> aid >> at >> a >> b >> c)) throw Error("data file: bad Angles row: " + r);
+                if (aid == 0) throw Error("Angle ID cannot be 0.");
+                if (aid < 1 || aid > n_angles || at < 1 || at > std::max(n_ant, 1L) || a < 1 || a > n_atoms ||
+                    b < 1 || b > n_atoms || c < 1 || c > n_atoms)
+                    throw Error("data file: angle entry out of range: " + r);
+                const size_t k = 4 * ((size_t)aid - 1);
+                s.h_anglelist[k] = (int)at;
+                s.h_anglelist[k + 1] = (int)a;
+                s.h_anglelist[k + 2] = (int)b;
+                s.h_anglelist[k + 3] = (int)c;
+            }
+            got_angles = true;
         } else {
             // coefficient sections: the deck's *_coeff commands set coefficients
             throw Error("data file: section '" + t + "' is not supported (use pair_coeff/bond_coeff in run.in)");
@@ -347,7 +368,8 @@ void read_data(System &s, const std::string &path)
     if (!got_atoms) throw Error("Atoms section is missing.");
     for (double m : s.masses)
         if (!(m > 0.0)) throw Error("All the masses are not set.");
-    if (s.atom_style == 1 && s.n_bonds > 0 && !got_bonds) throw Error("Bonds section is missing.");
+    if (molecular && s.n_bonds > 0 && !got_bonds) throw Error("Bonds section is missing.");
+    if (s.atom_style == 2 && s.n_angles > 0 && !got_angles) throw Error("Angles section is missing.");
     for (size_t k = 0; k < N; ++k)
         if (s.h_id[k] == 0) throw Error("data file: atom " + std::to_string(k + 1) + " is missing");
 }
@@ -357,17 +379,19 @@ void write_data_host(const System &s, const std::string &path)
     FILE *f = std::fopen(path.c_str(), "w");
     if (!f) throw Error("Error opening file: " + path);
     const int N = s.n_atoms;
-    const bool bonded = s.atom_style == 1;
+    const bool bonded = s.atom_style >= 1, angled = s.atom_style == 2;
     std::fprintf(f, "# Lammps format data file\n\n");
     std::fprintf(f, "%d atoms\n", N);
     if (bonded) std::fprintf(f, "%d bonds\n", s.n_bonds);
+    if (angled) std::fprintf(f, "%d angles\n", s.n_angles);
     std::fprintf(f, "%d atom types\n", s.n_atomtypes);
     if (bonded) std::fprintf(f, "%d bond types\n", s.n_bondtypes);
+    if (angled) std::fprintf(f, "%d angle types\n", s.n_angletypes);
     std::fprintf(f, "\n%.9f %.9f xlo xhi\n%.9f %.9f ylo yhi\n%.9f %.9f zlo zhi\n\n", s.box.xlo, s.box.xhi,
                  s.box.ylo, s.box.yhi, s.box.zlo, s.box.zhi);
     std::fprintf(f, "Masses\n\n");
     for (int t = 0; t < s.n_atomtypes; ++t) std::fprintf(f, "%d %.9f\n", t + 1, s.masses[(size_t)t]);
-    std::fprintf(f, "\nAtoms  # %s\n\n", bonded ? "bond" : "atomic");
+    std::fprintf(f, "\nAtoms  # %s\n\n", angled ? "angle" : bonded ? "bond" : "atomic");
     for (int k = 0; k < N; ++k) {
         const double *x = &s.h_pos[3 * (size_t)k];
         if (bonded)
@@ -386,6 +410,13 @@ void write_data_host(const System &s, const std::string &path)
         for (int b = 0; b < s.n_bonds; ++b)
             std::fprintf(f, "%d %d %d %d\n", b + 1, s.h_bondlist[3 * (size_t)b], s.h_bondlist[3 * (size_t)b + 1],
                          s.h_bondlist[3 * (size_t)b + 2]);
+    }
+    if (angled) {
+        std::fprintf(f, "\nAngles\n\n");
+        for (int a = 0; a < s.n_angles; ++a) {
+            const int *e = &s.h_anglelist[4 * (size_t)a];
+            std::fprintf(f, "%d %d %d %d %d\n", a + 1, e[0], e[1], e[2], e[3]);
+        }
     }
     std::fclose(f);
 }
@@ -668,6 +699,81 @@ static bool _gdpd_bond_harmonic_registered = []() {
     return true;
 }();
 
+// ---- Angle registry (force/angle.cu) ---------------------------------------------
+
+std::vector<std::unique_ptr<Angle>> &getAngleRegistry()
+{
+    static std::vector<std::unique_ptr<Angle>> r;
+    return r;
+}
+Angle *&getEnabledAngle()
+{
+    static Angle *p = nullptr;
+    return p;
+}
+void registerAngle(std::unique_ptr<Angle> instance) { getAngleRegistry().push_back(std::move(instance)); }
+void preprocessAngles(System &system)
+{
+    Angle *&en = getEnabledAngle();
+    en = nullptr;
+    for (auto &a : getAngleRegistry())
+        if (a->isEnabled(system)) {
+            if (en) throw Error("only one angle style can be enabled");
+            en = a.get();
+        }
+    if (!en) throw Error("no angle style is enabled (angle_style harmonic)");
+    en->print_angle_info(system);
+}
+void postprocessAngles() { getEnabledAngle() = nullptr; }
+void Angle_compute(System &system, unsigned int step)
+{
+    if (getEnabledAngle()) getEnabledAngle()->compute_force(system, step);
+}
+
+bool Angle_harmonic::isEnabled(System &s)
+{
+    bool enabled = false;
+    for_lines(s.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
+        if (key != "angle_style" || a.empty() || a[0] != "harmonic") return;
+        enabled = true;
+        if (a.size() != 1) illegal(l, "illegal angle_style harmonic command", "angle_style harmonic");
+    });
+    if (!enabled) return false;
+    std::vector<int> got(s.n_angletypes, 0);
+    angle_coeff.assign(2 * (size_t)s.n_angletypes, 0.0);
+    for_lines(s.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
+        if (key != "angle_coeff") return;
+        const char *fmt = "angle_coeff type k theta0";
+        if (a.size() != 3) illegal(l, "illegal angle_coeff command for angle_style harmonic", fmt);
+        const int t = (int)to_i(a[0], l, "type", fmt);
+        if (t < 1 || t > s.n_angletypes) throw Error("type exceeds n_angle_types");
+        got[t - 1] = 1;
+        angle_coeff[2 * (t - 1)] = to_f(a[1], l, "k", fmt);
+        angle_coeff[2 * (t - 1) + 1] = to_f(a[2], l, "theta0", fmt);
+    });
+    for (int g : got)
+        if (!g) throw Error("all angle_coeff are not set");
+    check(s, gdpd_set_angles(s.ctx, s.handle, s.n_angles, s.n_angletypes, s.h_anglelist.data(), angle_coeff.data()));
+    return true;
+}
+
+void Angle_harmonic::print_angle_info(System &s)
+{
+    for (int t = 0; t < s.n_angletypes; ++t)
+        std::cout << "   angle_coeff           " << t + 1 << " " << angle_coeff[2 * t] << " " << angle_coeff[2 * t + 1]
+                  << "\n";
+}
+
+void Angle_harmonic::compute_force(System &s, unsigned int step)
+{
+    check(s, gdpd_angle_compute(s.ctx, s.handle, step, (step % s.run_p.log_f) == 0));
+}
+
+static bool _gdpd_angle_harmonic_registered = []() {
+    registerAngle(std::make_unique<Angle_harmonic>());
+    return true;
+}();
+
 // ---- Fix registry (fix/fix.cu:5-195) -------------------------------------------
 
 std::unordered_map<std::string, FixConstructor> &getFixRegistry()
@@ -755,7 +861,10 @@ void Thermo::preprocess(System &s)
     if (file_path.empty()) return;
     file = std::fopen(file_path.c_str(), "w");
     if (!file) throw Error("cannot open thermo file: " + file_path);
-    if (s.atom_style == 1)
+    if (s.atom_style == 2)
+        std::fprintf(file, "%15s\t%10s\t%20s\t%20s\t%20s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\n", "step",
+                     "temp", "pair_pe", "bond_pe", "angle_pe", "pressure", "pxx", "pxy", "pxz", "pyy", "pyz", "pzz");
+    else if (s.atom_style == 1)
         std::fprintf(file, "%15s\t%10s\t%20s\t%20s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\t%12s\n", "step", "temp",
                      "pair_pe", "bond_pe", "pressure", "pxx", "pxy", "pxz", "pyy", "pyz", "pzz");
     else
@@ -771,9 +880,13 @@ void Thermo::postprocess(System &)
 
 void Thermo::write_row(const System &s, const std::vector<double> &r)
 {
-    // r: step, T, pair PE, bond PE, P, Pxx Pxy Pxz Pyy Pyz Pzz
+    // r: step, T, pair PE, bond PE, P, Pxx Pxy Pxz Pyy Pyz Pzz, angle PE
     if (!file) return;
-    if (s.atom_style == 1)
+    if (s.atom_style == 2)
+        std::fprintf(file,
+                     "%15u\t%10.6f\t%20.6f\t%20.6f\t%20.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\n",
+                     (unsigned)r[0], r[1], r[2], r[3], r[11], r[4], r[5], r[6], r[7], r[8], r[9], r[10]);
+    else if (s.atom_style == 1)
         std::fprintf(file, "%15u\t%10.6f\t%20.6f\t%20.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\t%12.6f\n",
                      (unsigned)r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]);
     else
@@ -792,7 +905,8 @@ void Thermo::process(System &s, unsigned int step)
     s.thermo_p.thermo_bond_pe = o[GDPD_THERMO_BOND_PE];
     s.thermo_p.thermo_pressure = o[GDPD_THERMO_P];
     for (int k = 0; k < 6; ++k) s.thermo_p.pressure_tensor[k] = o[GDPD_THERMO_PXX + k];
-    std::vector<double> r = {(double)step, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9]};
+    std::vector<double> r = {(double)step, o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9],
+                             o[GDPD_THERMO_ANGLE_PE]};
     rows.push_back(r);
     write_row(s, r);
     if (print)
@@ -805,7 +919,8 @@ void preprocess(System &system, Neigh_list &nl, Thermo &thermo)
 {
     nl.preprocess(system);
     preprocessPairs(system);
-    if (system.atom_style == 1) preprocessBonds(system);
+    if (system.atom_style >= 1) preprocessBonds(system);
+    if (system.atom_style == 2) preprocessAngles(system);
     preprocessFixes(system);
     thermo.preprocess(system);
 }
@@ -857,7 +972,8 @@ void zero_step(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump)
 {
     nl.build(system);
     Pair_compute(system, 0);
-    if (system.atom_style == 1) Bond_compute(system, 0);
+    if (system.atom_style >= 1) Bond_compute(system, 0);
+    if (system.atom_style == 2) Angle_compute(system, 0);
     Fix_post_force(system, 0);
     thermo.process(system, 0);
     // main/utils.cuh:200-204
@@ -878,7 +994,8 @@ void simulation(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump)
             }
         }
         Pair_compute(system, step);
-        if (system.atom_style == 1) Bond_compute(system, step);
+        if (system.atom_style >= 1) Bond_compute(system, step);
+        if (system.atom_style == 2) Angle_compute(system, step);
         Fix_post_force(system, step);
         Fix_final_integrate(system, step);
         Fix_end_of_step(system, step);
@@ -941,7 +1058,8 @@ void run_fused(System &system, Thermo &thermo)
     thermo.rows.clear();
     for (int k = 0; k < n; ++k) {
         const double *o = rows.data() + (size_t)k * GDPD_THERMO_LEN;
-        thermo.rows.push_back({o[GDPD_THERMO_STEP], o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9]});
+        thermo.rows.push_back({o[GDPD_THERMO_STEP], o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9],
+                               o[GDPD_THERMO_ANGLE_PE]});
     }
 }
 

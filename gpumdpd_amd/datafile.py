@@ -19,17 +19,22 @@ def write_data(cfg, path, title="gdpd data file"):
     pos = np.asarray(cfg["pos"])[order]
     vel = np.asarray(cfg["vel"])[order]
     bonded = "bonds" in cfg
+    angled = "angles" in cfg
     lines = [f"# {title}", "", f"{n} atoms"]
     if bonded:
         lines.append(f"{len(cfg['bonds'])} bonds")
+    if angled:
+        lines.append(f"{len(cfg['angles'])} angles")
     lines.append(f"{cfg['ntypes']} atom types")
     if bonded:
         lines.append(f"{cfg['n_bond_types']} bond types")
+    if angled:
+        lines.append(f"{cfg['n_angle_types']} angle types")
     lo, hi = cfg["lo"], cfg["hi"]
     lines += ["", f"{lo[0]:.17g} {hi[0]:.17g} xlo xhi", f"{lo[1]:.17g} {hi[1]:.17g} ylo yhi",
               f"{lo[2]:.17g} {hi[2]:.17g} zlo zhi", "", "Masses", ""]
     lines += [f"{t + 1} {m:.17g}" for t, m in enumerate(cfg["masses"])]
-    lines += ["", "Atoms  # " + ("bond" if bonded else "atomic"), ""]
+    lines += ["", "Atoms  # " + ("angle" if angled else "bond" if bonded else "atomic"), ""]
     mol = cfg.get("mol")
     for k in range(n):
         x = pos[k]
@@ -44,6 +49,10 @@ def write_data(cfg, path, title="gdpd data file"):
         lines += ["", "Bonds", ""]
         b = np.asarray(cfg["bonds"])
         lines += [f"{i + 1} {int(b[i, 0])} {int(b[i, 1])} {int(b[i, 2])}" for i in range(len(b))]
+    if angled:
+        lines += ["", "Angles", ""]
+        a = np.asarray(cfg["angles"])
+        lines += [f"{i + 1} {int(a[i, 0])} {int(a[i, 1])} {int(a[i, 2])} {int(a[i, 3])}" for i in range(len(a))]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -54,14 +63,14 @@ def read_data(path):
     with open(path) as f:
         raw = f.read().split("\n")
     strip = [l.split("#")[0].strip() for l in raw]
-    out = {"mol": None, "vel": None, "bonds": None}
+    out = {"mol": None, "vel": None, "bonds": None, "angles": None}
     i = 1
     counts = {}
-    sections = {"Masses", "Atoms", "Velocities", "Bonds"}
+    sections = {"Masses", "Atoms", "Velocities", "Bonds", "Angles"}
     lo, hi = [0.0] * 3, [0.0] * 3
     while i < len(strip) and strip[i] not in sections:
         t = strip[i].split()
-        if len(t) >= 2 and t[-1] in ("atoms", "bonds"):
+        if len(t) >= 2 and t[-1] in ("atoms", "bonds", "angles"):
             counts[t[-1]] = int(t[0])
         elif len(t) >= 3 and t[-1] == "types":
             counts[t[-2] + " types"] = int(t[0])
@@ -87,8 +96,9 @@ def read_data(path):
             i += 1
             continue
         if name == "Atoms":
-            style_bond = "bond" in raw[i]
-        count = {"Masses": out["ntypes"], "Atoms": n, "Velocities": n, "Bonds": counts.get("bonds", 0)}[name]
+            style_bond = "bond" in raw[i] or "angle" in raw[i]
+        count = {"Masses": out["ntypes"], "Atoms": n, "Velocities": n, "Bonds": counts.get("bonds", 0),
+                 "Angles": counts.get("angles", 0)}[name]
         r, i = rows(i + 1, count)
         a = np.array(r, dtype=np.float64) if r else np.zeros((0, 4))
         if name == "Masses":
@@ -106,6 +116,9 @@ def read_data(path):
         elif name == "Bonds":
             a = a[np.argsort(a[:, 0])]
             out["bonds"] = a[:, 1:4].astype(np.int64)
+        elif name == "Angles":
+            a = a[np.argsort(a[:, 0])]
+            out["angles"] = a[:, 1:5].astype(np.int64)
     return out
 
 
@@ -114,8 +127,10 @@ def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None, dump=No
     neighbor, timestep, thermo, run, read_data, group and fix nve); dump is
     the argument string of a dump command, e.g. "0 5 wrapped force_yes vel_yes"."""
     bonded = "bonds" in cfg
+    angled = "angles" in cfg
     nt = cfg["ntypes"]
-    lines = ["# generated deck", f"atom_style {'bond' if bonded else 'atomic'}",
+    style = "angle" if angled else "bond" if bonded else "atomic"
+    lines = ["# generated deck", f"atom_style {style}",
              f"neighbor {cfg['global_cut']} {cfg['skin']} {cfg['max_rho']} 1", f"timestep {cfg['dt']}",
              f"thermo {log_every}", f"run {nsteps}", f"read_data {data_name}"]
     if write_name:
@@ -143,6 +158,10 @@ def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None, dump=No
         lines.append("bond_style harmonic")
         bc = np.asarray(cfg["bond_coeff"], dtype=np.float64).reshape(-1, 2)
         lines += [f"bond_coeff {t + 1} {bc[t, 0]:.17g} {bc[t, 1]:.17g}" for t in range(len(bc))]
+    if angled:
+        lines.append("angle_style harmonic")
+        ac = np.asarray(cfg["angle_coeff"], dtype=np.float64).reshape(-1, 2)
+        lines += [f"angle_coeff {t + 1} {ac[t, 0]:.17g} {ac[t, 1]:.17g}" for t in range(len(ac))]
     lines += ["group all type " + " ".join(str(t) for t in range(1, nt + 1)), "fix md all nve"]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

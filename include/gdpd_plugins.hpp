@@ -52,12 +52,12 @@ struct Thermo_p {
 // The host side of system/system.cuh:112-154; device arrays live in libgdpd.
 struct System {
     std::string input;            // run.in text with comments and blank lines removed
-    int atom_style = 0;           // 0 atomic, 1 bond
-    int n_atoms = 0, n_atomtypes = 0, n_bonds = 0, n_bondtypes = 0;
+    int atom_style = 0;           // 0 atomic, 1 bond, 2 angle
+    int n_atoms = 0, n_atomtypes = 0, n_bonds = 0, n_bondtypes = 0, n_angles = 0, n_angletypes = 0;
     Box box;
     Run_p run_p;
     Thermo_p thermo_p;
-    std::vector<int> h_id, h_type, h_bondlist;
+    std::vector<int> h_id, h_type, h_bondlist, h_anglelist;
     std::vector<int> h_mol_id;    // molecule ids (atom_style bond), index = id - 1
     std::vector<double> h_pos, h_vel, masses;
     bool has_velocities = false;  // the data file had a Velocities section
@@ -157,6 +157,33 @@ public:
     std::vector<double> bond_coeff;
 };
 
+// ---- Angle (force/angle.cuh, force/angle.cu; atom_style angle) ---------------
+class Angle {
+public:
+    virtual ~Angle() = default;
+    virtual bool isEnabled(System &system) = 0;
+    virtual void print_angle_info(System &system) = 0;
+    virtual std::string getName() = 0;
+    virtual void compute_force(System &system, unsigned int step) = 0;
+};
+std::vector<std::unique_ptr<Angle>> &getAngleRegistry();
+Angle *&getEnabledAngle();
+void registerAngle(std::unique_ptr<Angle> instance);
+void preprocessAngles(System &system);
+void postprocessAngles();
+void Angle_compute(System &system, unsigned int step);
+
+// force/angle_harmonic.cu:19-151, 459-...: `angle_style harmonic`,
+// `angle_coeff type k theta0` (degrees), E = k (theta - theta0)^2
+class Angle_harmonic : public Angle {
+public:
+    bool isEnabled(System &system) override;
+    void print_angle_info(System &system) override;
+    std::string getName() override { return "harmonic"; }
+    void compute_force(System &system, unsigned int step) override;
+    std::vector<double> angle_coeff;
+};
+
 // ---- Fix (fix/fix.cuh:7-59) -------------------------------------------------
 class Fix {
 public:
@@ -216,7 +243,7 @@ public:
     void process(System &system, unsigned int step);
     void postprocess(System &system);
     void write_row(const System &system, const std::vector<double> &row);
-    std::vector<std::vector<double>> rows;  // step, T, pair pe, bond pe, P, P6
+    std::vector<std::vector<double>> rows;  // step, T, pair pe, bond pe, P, P6, angle pe
     bool print = false;
     std::string file_path;
     FILE *file = nullptr;

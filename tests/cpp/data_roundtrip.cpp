@@ -1,6 +1,6 @@
 // Reads a LAMMPS-style data file with the shells' read_data and writes it back
 // with write_data_host (host only, no device).  Usage:
-//   data_roundtrip atomic|bond in.data out.data
+//   data_roundtrip atomic|bond|angle in.data out.data
 // Exit status 0 and "ok <atoms> <bonds>" on success; 1 and the error message
 // on stderr when read_data rejects the file.
 #include <cstdio>
@@ -11,11 +11,12 @@
 int main(int argc, char **argv)
 {
     if (argc != 4) {
-        std::fprintf(stderr, "usage: data_roundtrip atomic|bond in.data out.data\n");
+        std::fprintf(stderr, "usage: data_roundtrip atomic|bond|angle in.data out.data\n");
         return 2;
     }
     gdpd_host::System s;
-    s.atom_style = std::string(argv[1]) == "bond" ? 1 : 0;
+    const std::string style = argv[1];
+    s.atom_style = style == "angle" ? 2 : style == "bond" ? 1 : 0;
     try {
         gdpd_host::read_data(s, argv[2]);
         gdpd_host::write_data_host(s, argv[3]);

@@ -1,7 +1,7 @@
 // Runs a reference deck (run.in + read_data file) through the C++ shells the
 // way main/main.cu:15-72 does.  Usage:
 //   deck_runner run.in [fused|hook] [f32|f64]
-// Prints one "thermo <step> <T> <pair PE> <bond PE> <P>" line per thermo row.
+// Prints one "thermo <step> <T> <pair PE> <bond PE> <P> <angle PE>" line per thermo row.
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -37,7 +37,7 @@ int main(int argc, char **argv)
         return 1;
     }
     for (const auto &r : thermo.rows)
-        std::printf("thermo %.0f %.17g %.17g %.17g %.17g\n", r[0], r[1], r[2], r[3], r[4]);
+        std::printf("thermo %.0f %.17g %.17g %.17g %.17g %.17g\n", r[0], r[1], r[2], r[3], r[4], r[11]);
     gdpd_host::postprocessPairs();
     gdpd_host::release(s);
     return 0;

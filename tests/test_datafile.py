@@ -27,6 +27,12 @@ def bonded_cfg():
     return cfg
 
 
+def angle_cfg():
+    cfg = configs.config_angles(n_chains=60, box=8.0)
+    cfg["mol"] = 1 + (np.arange(cfg["N"]) // 16)
+    return cfg
+
+
 def compare(d, cfg, tol):
     order = np.argsort(cfg["id"])
     assert d["n"] == cfg["N"]
@@ -70,12 +76,26 @@ def test_positions_outside_the_box_are_wrapped(tmp_path):
     compare(datafile.read_data(dst), cfg, 1e-8)
 
 
+def test_roundtrip_angles(tmp_path):
+    cfg = angle_cfg()
+    src, dst = str(tmp_path / "in.data"), str(tmp_path / "out.data")
+    datafile.write_data(cfg, src)
+    r = roundtrip("angle", src, dst)
+    assert r.returncode == 0, r.stderr
+    d = datafile.read_data(dst)
+    compare(d, cfg, 6e-10)
+    assert np.array_equal(d["bonds"], np.asarray(cfg["bonds"]))
+    assert np.array_equal(d["angles"], np.asarray(cfg["angles"]))
+    # an angle file is refused under atom_style bond
+    assert roundtrip("bond", src, dst).returncode == 1
+
+
 @pytest.mark.parametrize("edit,message", [
     (lambda t: t.replace("Masses", "Massless"), "Massless"),
     (lambda t: t.replace("Atoms  # atomic", "Atoms  # bond"), "inconsistent"),
     (lambda t: t.replace("\n2 1 ", "\n1 1 ", 1), "duplicate atom id"),
     (lambda t: t.replace("\n1 1 ", "\n0 1 ", 1), "Atom ID cannot be 0"),
-    (lambda t: t.replace(" atom types", " atom types\n3 angles"), "angle"),
+    (lambda t: t.replace(" atom types", " atom types\n3 angles"), "angles but atom_style"),
 ])
 def test_reference_error_cases(tmp_path, edit, message):
     cfg = configs.scaled_dpd(300, seed=5)
@@ -102,9 +122,9 @@ def run_deck(tmp_path, cfg, nsteps, log_every, mode, prec):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["dpd", "bonded"])
+@pytest.mark.parametrize("name", ["dpd", "bonded", "angles"])
 def test_deck_through_shells_matches_direct_run(tmp_path, name):
-    cfg = configs.scaled_dpd(3000, seed=6) if name == "dpd" else bonded_cfg()
+    cfg = {"dpd": lambda: configs.scaled_dpd(3000, seed=6), "bonded": bonded_cfg, "angles": angle_cfg}[name]()
     nsteps, every = 20, 5
     print(f"case: deck {name}", flush=True)
     d_fused, rows_fused = run_deck(tmp_path, cfg, nsteps, every, "fused", "f64")
@@ -121,10 +141,10 @@ def test_deck_through_shells_matches_direct_run(tmp_path, name):
     for d in (d_fused, d_hook):
         assert np.allclose(d["pos"], ref["pos"], atol=6e-10, rtol=0)
         assert np.allclose(d["vel"], ref["vel"], atol=6e-10, rtol=0)
-    # step, T, pair PE, bond PE, P of every thermo row
-    assert rows_fused.shape == (nsteps // every + 1, 5)
+    # step, T, pair PE, bond PE, P, angle PE of every thermo row
+    assert rows_fused.shape == (nsteps // every + 1, 6)
     assert np.allclose(rows_fused[:, 0], ref_rows[:, 11])
-    assert np.allclose(rows_fused[:, 1:], ref_rows[:, [0, 1, 2, 3]], rtol=1e-12, atol=1e-12)
+    assert np.allclose(rows_fused[:, 1:], ref_rows[:, [0, 1, 2, 3, 12]], rtol=1e-12, atol=1e-12)
     assert np.allclose(rows_hook, rows_fused, rtol=1e-11, atol=1e-11)
 
 
