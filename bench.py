@@ -76,7 +76,7 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
     import gpumdpd_amd as g
     from gpumdpd_amd import serve, tune
     t = time.perf_counter()
-    tuned = {st: tune.tune_brick(ctx, st, g.PREC_F32) for st in ("dpd", "mdpd")}
+    tuned = tune.tune_classes(ctx, ("dpd", "mdpd"), g.PREC_F32)
     log(f"bench: tuned bricks {tuned} in {time.perf_counter() - t:.1f}s")
     reqs = serve.make_requests(5, n=n_req, steps=steps)
     serve.prepare_inputs(reqs)
@@ -92,7 +92,8 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
             "step_latency_p50_ms": m["step_latency_p50_ms"], "step_latency_p99_ms": m["step_latency_p99_ms"],
             "request_latency_p50_s": m["request_latency_p50_s"], "request_latency_p99_s": m["request_latency_p99_s"],
             "bead_steps_per_s": m["bead_steps_per_s"], "makespan_s": m["makespan_s"], "requests": n_req,
-            "steps_per_request": steps, "slots": slots, "chunk_steps": chunk, "tuned_bricks": tuned,
+            "steps_per_request": steps, "slots": slots, "chunk_steps": chunk,
+            "tuned_bricks": {f"{k[0]}/{'small' if k[1] == 0 else 'large'}": v for k, v in tuned.items()},
             "plain": {"replica_steps_per_s": p["replica_steps_per_s"], "bead_steps_per_s": p["bead_steps_per_s"],
                       "step_latency_p50_ms": p["step_latency_p50_ms"], "step_latency_p99_ms": p["step_latency_p99_ms"],
                       "sample": f"every {plain_every}th request ({len(sample)} of {n_req}), one at a time, "
@@ -126,7 +127,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=128)
-    ap.add_argument("--serve-chunk", type=int, default=16)
+    ap.add_argument("--serve-chunk", type=int, default=32)
     ap.add_argument("--plain-every", type=int, default=8)
     args = ap.parse_args()
     if args.copy_calibration:

@@ -84,8 +84,10 @@ def _system_bytes(cfg):
 
 def _create(ctx, req, precision, log_every, tuned):
     import gpumdpd_amd as g
+    from .tune import size_class
     s = g.system_from_config(ctx, req.cfg, precision=precision, log_every=log_every, upload=False)
-    b = tuned.get(req.style)
+    # tuned: brick shape by (style, size class) or by style
+    b = tuned.get((req.style, size_class(req.n)), tuned.get(req.style))
     if b:
         s.set_tuning(brick=b)
     c = req.cfg
@@ -100,7 +102,7 @@ def _percentiles(lat):
     return float(np.percentile(a, 50)), float(np.percentile(a, 99))
 
 
-def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=16, precision=L.PREC_F32, log_every=100,
+def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=32, precision=L.PREC_F32, log_every=100,
           tuned=None, keep=None, realtime=True):
     """Run the stream with continuous batching.  `keep`: request ids whose
     final state (pos, vel, thermo rows) is returned.  Returns a metrics dict."""
@@ -186,7 +188,7 @@ def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=16, precision=L.PREC
             "requests": len(reqs), "replica_steps": total_steps, "kept": kept}
 
 
-def serve_plain(ctx, reqs, chunk=16, precision=L.PREC_F32, log_every=100, tuned=None, keep=None,
+def serve_plain(ctx, reqs, chunk=32, precision=L.PREC_F32, log_every=100, tuned=None, keep=None,
                 realtime=True):
     """Plain baseline: the same requests one at a time in arrival order."""
     tuned = tuned or {}

@@ -46,10 +46,28 @@ def _time_force(ctx, cfg, brick, precision, steps):
         s.destroy()
 
 
+# replica size classes for serving: tiles are tuned on one sample per class
+SIZE_CLASSES = [(0, 12000), (30000, 120000)]  # (lower bound of N, sample size)
+
+
+def size_class(n):
+    k = 0
+    for i, (lo, _) in enumerate(SIZE_CLASSES):
+        if n >= lo:
+            k = i
+    return k
+
+
+def tune_classes(ctx, styles, precision, verbose=False):
+    """Brick shape per (style, size class), for serve.serve's `tuned` table."""
+    return {(st, k): tune_brick(ctx, st, precision, n_sample=n, verbose=verbose)
+            for st in styles for k, (_, n) in enumerate(SIZE_CLASSES)}
+
+
 def tune_brick(ctx, style, precision, n_sample=60000, steps=6, shapes=None, verbose=False):
     """Fastest brick shape for replicas of `style` ("dpd" at rho=3, "mdpd" at
     rho=6) in `precision`, from force-kernel time on an n_sample-bead replica."""
-    key = (style, precision)
+    key = (style, precision, n_sample)
     if key in _CACHE:
         return _CACHE[key]
     cfg = configs.replica_system(style, n_sample, 12345)
