@@ -15,6 +15,11 @@ ERR_NAMES = {1: "GDPD_ERR_ARG", 2: "GDPD_ERR_HIP", 3: "GDPD_ERR_OVERFLOW", 4: "G
 PREC_F32, PREC_F64 = 0, 1
 STYLE_DPD, STYLE_MDPD = 1, 2
 THERMO_LEN = 13
+FIX_KINDS = {"nve": 1, "move": 2, "wall_reflect": 3, "two_wall": 4, "spring": 5, "recenter": 6, "rvcm": 7}
+COMPUTE_KINDS = {"temp": 1, "msd": 2, "rho": 3, "tf": 4, "wall_contact": 5}
+HOOKS = {"initial_integrate": 0, "post_integrate": 1, "post_force": 2, "final_integrate": 3, "end_of_step": 4}
+MAX_FIXES = MAX_COMPUTES = 8
+RECORD_LEN, RECORD_CAP = 6, 1024
 
 
 class SystemDesc(C.Structure):
@@ -24,6 +29,20 @@ class SystemDesc(C.Structure):
         ("dt", C.c_double), ("global_cut", C.c_double), ("skin", C.c_double),
         ("max_rho", C.c_double), ("check_every", C.c_int), ("log_every", C.c_int),
         ("precision", C.c_int),
+    ]
+
+
+class Fix(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("group", C.c_uint32), ("group2", C.c_uint32), ("group3", C.c_uint32),
+        ("every", C.c_int), ("dir", C.c_int), ("flags", C.c_int), ("p", C.c_double * 5),
+    ]
+
+
+class Compute(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("group", C.c_uint32), ("group2", C.c_uint32), ("every", C.c_int),
+        ("dir", C.c_int), ("side", C.c_int), ("p", C.c_double * 6),
     ]
 
 
@@ -62,6 +81,11 @@ PROTOTYPES = {
     "gdpd_finish": (_I, [_P, _I]),
     "gdpd_thermo_rows": (_I, [_P, _I, _PD, _I, _PI]),
     "gdpd_get_step": (_I, [_P, _I, C.POINTER(C.c_int64)]),
+    "gdpd_set_fixes": (_I, [_P, _I, C.POINTER(Fix), _I]),
+    "gdpd_fix_hook": (_I, [_P, _I, _I, _I, C.c_uint32, _PD]),
+    "gdpd_set_computes": (_I, [_P, _I, C.POINTER(Compute), _I]),
+    "gdpd_compute_eval": (_I, [_P, _I, _I, _PD]),
+    "gdpd_records": (_I, [_P, _I, _I, _PD, _I, _PI]),
     "gdpd_batch_advance": (_I, [_P, _PI, _I, _I]),
     "gdpd_set_tuning": (_I, [_P, _I, _I, _I, _I, _I, _I]),
     "gdpd_get_tuning": (_I, [_P, _I, _PI]),

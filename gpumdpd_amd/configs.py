@@ -190,3 +190,33 @@ def config_angles(n_chains=1500, box=20.0, seed=31):
     cfg["angle_coeff"] = np.array([3.0, 180.0, 2.0, 120.0])
     cfg["name"] = f"angle melt {cfg['N']}"
     return cfg
+
+
+def config_wetting(seed=31, n_fluid=3000, box=(10.0, 10.0, 14.0)):
+    """Small wetting-study geometry for the group fixes and computes: a DPD
+    fluid (type 1) between two frozen wall slabs one cell thick at the bottom
+    (type 2) and top (type 3) of z, plus a 20-bead probe cluster (type 4) in
+    the fluid.  rho = 3 everywhere; the fluid starts inside z in [1.2, L-1.2).
+    """
+    box = np.asarray(box, dtype=np.float64)
+    area = box[0] * box[1]
+    n_wall = int(round(3.0 * area * 1.0))
+    u = uniform(substream(seed, 0), 3 * (n_fluid + 2 * n_wall + 20)).reshape(-1, 3)
+    fl = u[:n_fluid] * np.array([box[0], box[1], box[2] - 2.4]) + np.array([0.0, 0.0, 1.2])
+    wlo = u[n_fluid:n_fluid + n_wall] * np.array([box[0], box[1], 1.0])
+    whi = u[n_fluid + n_wall:n_fluid + 2 * n_wall] * np.array([box[0], box[1], 1.0]) + np.array([0, 0, box[2] - 1.0])
+    probe = u[n_fluid + 2 * n_wall:] * 1.5 + box / 2.0 - 0.75
+    pos = np.concatenate([fl, wlo, whi, probe])
+    typ = np.concatenate([np.full(n_fluid, 1), np.full(n_wall, 2), np.full(n_wall, 3), np.full(20, 4)]).astype(np.int32)
+    n = len(pos)
+    vel = maxwell(substream(seed, 1), n)
+    vel[typ == 2] = 0.0
+    vel[typ == 3] = 0.0
+    table = {}
+    for ti in range(1, 5):
+        for tj in range(ti, 5):
+            table[(ti, tj)] = (25.0 if ti == tj else 40.0, 4.5, 1.0)
+    return {"N": n, "ntypes": 4, "lo": np.zeros(3), "hi": box.copy(), "id": np.arange(1, n + 1, dtype=np.int32),
+            "type": typ, "pos": pos, "vel": vel, "masses": np.array([1.0, 1.0, 1.0, 2.0]),
+            "style": "dpd", "temp": 1.0, "rc": 1.0, "saru_seed": 77, "dt": 0.02, "skin": 0.3,
+            "global_cut": 1.0, "max_rho": 8.0, "coeff": dpd_coeff(4, table), "name": "wetting walls + probe"}

@@ -80,6 +80,13 @@ struct Sys {
     double *nl_last = nullptr;
     bool nl_built = false;
     int64_t bytes = 0;
+    // group fixes and computes (the given fixes, then an nve on every type when
+    // none is given) and the host mirror of the step counter on their path
+    std::vector<gdpd_fix> fixes;
+    std::vector<gdpd_compute> computes;
+    int type_count[kMaxTypes] = {0};
+    bool fix_path = false;
+    long long fix_step = 0;
 };
 
 } // namespace
@@ -623,6 +630,186 @@ int advance_tables(gdpd_ctx *ctx, Tables &T, Graph &G, int nsteps, int graph_ste
     return GDPD_OK;
 }
 
+
+// ---- group fixes and computes --------------------------------------------------
+
+unsigned all_types(const Sys *s) { return s->h.ntypes >= 32 ? ~0u : ((1u << s->h.ntypes) - 1u); }
+
+// bead count and mass of every fix's and compute's group (type masks)
+void refresh_groups(Sys *s)
+{
+    RepDev &h = s->h;
+    auto stats = [&](unsigned g, int &cnt, double &mass) {
+        cnt = 0;
+        mass = 0.0;
+        for (int t = 0; t < h.ntypes; ++t)
+            if ((g >> t) & 1u) {
+                cnt += s->type_count[t];
+                mass += s->type_count[t] * h.mass[t];
+            }
+    };
+    for (int k = 0; k < h.nfix; ++k) stats(h.fix[k].group, h.fix[k].gcount, h.fix[k].gmass);
+    for (int k = 0; k < h.ncomp; ++k) stats(h.comp[k].group, h.comp[k].gcount, h.comp[k].gmass);
+}
+
+// Device fix list = the caller's fixes in order; with none, nve on every bead
+// (the default of a system that never set fixes).  Anything beyond that one
+// nve, or any compute, runs the step as the hook sequence (fix_iteration).
+void finish_fix_setup(Sys *s)
+{
+    RepDev &h = s->h;
+    const unsigned all = all_types(s);
+    h.nfix = (int)s->fixes.size();
+    for (int k = 0; k < h.nfix; ++k) {
+        const gdpd_fix &f = s->fixes[k];
+        FixDev &F = h.fix[k];
+        F = FixDev{};
+        F.kind = f.kind;
+        F.group = f.group;
+        F.group2 = f.group2;
+        F.group3 = f.group3;
+        F.every = f.every;
+        F.dir = f.dir;
+        F.flags = f.flags;
+        for (int q = 0; q < 5; ++q) F.p[q] = f.p[q];
+    }
+    if (h.nfix == 0) {
+        h.nfix = 1;
+        h.fix[0] = FixDev{};
+        h.fix[0].kind = GDPD_FIX_NVE;
+        h.fix[0].group = all;
+    }
+    s->fix_path = !(h.nfix == 1 && h.fix[0].kind == GDPD_FIX_NVE && h.fix[0].group == all) || h.ncomp > 0;
+    refresh_groups(s);
+    s->graph.release();
+}
+
+unsigned fix_mask(const Sys *s, int kind)
+{
+    unsigned m = 0;
+    for (int k = 0; k < s->h.nfix; ++k)
+        if (s->h.fix[k].kind == kind) m |= 1u << k;
+    return m;
+}
+
+// fixes of `kind` whose every-N schedule fires at step n
+unsigned fix_firing(const Sys *s, int kind, long long n)
+{
+    unsigned m = 0;
+    for (int k = 0; k < s->h.nfix; ++k) {
+        const FixDev &F = s->h.fix[k];
+        if (F.kind == kind && F.every > 0 && n % F.every == 0) m |= 1u << k;
+    }
+    return m;
+}
+
+unsigned comp_firing(const Sys *s, long long n, int kind)
+{
+    unsigned m = 0;
+    for (int k = 0; k < s->h.ncomp; ++k) {
+        const CompDev &C = s->h.comp[k];
+        if ((kind == 0 || C.kind == kind) && C.every > 0 && n % C.every == 0) m |= 1u << k;
+    }
+    return m;
+}
+
+int check_groups(gdpd_ctx *ctx, const Sys *s)
+{
+    for (int k = 0; k < s->h.nfix; ++k) {
+        const FixDev &F = s->h.fix[k];
+        if ((F.kind == GDPD_FIX_SPRING || F.kind == GDPD_FIX_RECENTER || F.kind == GDPD_FIX_RVCM) && !(F.gmass > 0.0))
+            return fail(ctx, GDPD_ERR_STATE, "fix %d: its group holds no beads", k);
+        if (F.kind == GDPD_FIX_TWO_WALL) {
+            for (unsigned g : {F.group2, F.group3}) {
+                int n = 0;
+                for (int t = 0; t < s->h.ntypes; ++t)
+                    if ((g >> t) & 1u) n += s->type_count[t];
+                if (n == 0) return fail(ctx, GDPD_ERR_STATE, "fix %d (two_wall): a wall group holds no beads", k);
+            }
+        }
+    }
+    for (int k = 0; k < s->h.ncomp; ++k)
+        if (s->h.comp[k].gcount <= 0) return fail(ctx, GDPD_ERR_STATE, "compute %d: its group holds no beads", k);
+    return GDPD_OK;
+}
+
+// keys, scan and scatter for the current positions (no in-cell order)
+void rebin_fast(gdpd_ctx *ctx, Tables &T)
+{
+    launch_bin_keys(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+    launch_scan(ctx->d_reps, T.scan, T.n_scan, T.rep_chunk_off, T.rep_list, T.n_reps, T.chunk_sums, ctx->stream);
+    launch_scatter(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
+}
+
+// One step of a system with group fixes or computes, in the reference's hook
+// order (main/utils.cuh:166-314) with every reduction on the device:
+//   force at x(n): pair (per-bead PE/virial on log steps), bonds, angles;
+//   post_force (spring); final_integrate (nve) unless step 0; end_of_step
+//   (rvcm, recenter) unless step 0; thermo record; compute records;
+//   then, unless this is the last step, initial_integrate and post_integrate
+//   of step n + 1 and its binning.
+int fix_iteration(gdpd_ctx *ctx, Sys *s, bool final_kick, bool end_of_step, bool next_initial)
+{
+    RepDev &h = s->h;
+    Tables &T = s->tab;
+    hipStream_t st = ctx->stream;
+    const long long n = s->fix_step;
+    const long long N = h.N;
+    const bool log = h.log_every > 0 && n % h.log_every == 0;
+    const unsigned spring = fix_mask(s, GDPD_FIX_SPRING), nve = fix_mask(s, GDPD_FIX_NVE),
+                   two_wall = fix_mask(s, GDPD_FIX_TWO_WALL);
+    const unsigned recenter = end_of_step ? fix_firing(s, GDPD_FIX_RECENTER, n) : 0u;
+    const unsigned rvcm = end_of_step ? fix_firing(s, GDPD_FIX_RVCM, n) : 0u;
+    const unsigned spring_rec = fix_firing(s, GDPD_FIX_SPRING, n);
+    const unsigned cfire = comp_firing(s, n, 0), ctf = comp_firing(s, n, GDPD_COMPUTE_TF),
+                   ccontact = comp_firing(s, n, GDPD_COMPUTE_WALL_CONTACT);
+    const unsigned cend = cfire & ~ctf;
+    launch_fix_clear(ctx->d_reps, T.rep_list, T.n_reps, st);
+    if (spring | recenter) launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, spring | recenter, 0u, 0, st);
+    if (log) {
+        launch_fill_f64(h.pe, 0.0, N, st);
+        launch_fill_f64(h.vir, 0.0, 6 * N, st);
+    }
+    int rc;
+    if ((rc = force_launch(ctx, T, kModeForceOnly | (log ? kModeLogForce : 0)))) return rc;
+    if (log) {
+        launch_fix_to_double(h.pe, N, h.precision, st);
+        launch_fix_to_double(h.vir, 6 * N, h.precision, st);
+    }
+    if (h.nbonds) launch_bond_hook(ctx->d_reps, T.beads, T.n_beads, log, st);
+    if (h.nangles) launch_angle_hook(ctx->d_reps, T.beads, T.n_beads, log, st);
+    if (spring) launch_fix_post_force(ctx->d_reps, T.beads, T.n_beads, spring, st);
+    if (final_kick) launch_fix_final(ctx->d_reps, T.beads, T.n_beads, nve, st);
+    if (rvcm | ctf) launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, rvcm, ctf, 1, st);
+    if (rvcm | recenter) launch_fix_end(ctx->d_reps, T.beads, T.n_beads, rvcm | recenter, st);
+    if (log) launch_thermo_arrays(ctx->d_reps, T.beads, T.n_beads, nullptr, st);
+    if (cend) launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, 0u, cend, 2, st);
+    if (ccontact) launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, 0u, ccontact, 3, st);
+    if (cfire | spring_rec) launch_fix_records(ctx->d_reps, T.rep_list, T.n_reps, spring_rec, cfire, n, st);
+    launch_step_end(ctx->d_reps, T.rep_list, T.n_reps, 0, 1, 1, st, 0);
+    s->fix_step = n + 1;
+    if (next_initial) {
+        launch_fix_initial(ctx->d_reps, T.beads, T.n_beads, (1u << h.nfix) - 1u, st);
+        if (two_wall) launch_fix_two_wall(ctx->d_reps, T.beads, T.n_beads, two_wall, st);
+        rebin_fast(ctx, T);
+        // the physical re-sort moves x, v and the bead tables but not the force:
+        // safe here, the next use of the force follows the next force evaluation
+        if ((rc = check_launch(ctx))) return rc;
+        return maybe_resort(ctx, s, T, 1);
+    }
+    return check_launch(ctx);
+}
+
+int fix_zero_step(gdpd_ctx *ctx, Sys *s, bool next_initial)
+{
+    int rc;
+    if ((rc = check_groups(ctx, s))) return rc;
+    if ((rc = ensure_cells(ctx, s))) return rc;
+    if ((rc = set_state_step(ctx, s, 0, true))) return rc;
+    launch_fill_i32(s->h.rec_n, 0, kRecSlots, ctx->stream);
+    s->fix_step = 0;
+    return fix_iteration(ctx, s, false, false, next_initial);
+}
 } // namespace
 
 // =============================================================================
@@ -743,6 +930,10 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     ALLOC(h.rho, N);
     ALLOC(h.iscratch, 3 * (size_t)N);
     ALLOC(h.state, 1);
+    ALLOC(h.racc, (size_t)kRecSlots * kAccPer * 2);
+    ALLOC(h.rext, (size_t)kRecSlots * 2);
+    ALLOC(h.rec_rows, (size_t)kRecSlots * kRecCap * kRecLen);
+    ALLOC(h.rec_n, (size_t)kRecSlots);
     h.thermo_cap = 1024;
     ALLOC(h.thermo_rows, (size_t)h.thermo_cap * kThermoRowLen);
     ALLOC(h.thermo_bead, (size_t)((N + kBlock - 1) / kBlock) * kThermoParts);
@@ -751,6 +942,9 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
 #undef ALLOC
     h.scratch = nullptr;
     HIPCHK(hipMemsetAsync(h.bond_start, 0, sizeof(int) * ((size_t)N + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(h.rec_n, 0, sizeof(int) * kRecSlots, ctx->stream));
+    h.x0uw = nullptr;
+    h.nfix = h.ncomp = 0;
     HIPCHK(hipMemsetAsync(h.force, 0, sizeof(double) * 3 * (size_t)N, ctx->stream));
     HIPCHK(hipMemsetAsync(h.pe, 0, sizeof(double) * (size_t)N, ctx->stream));
     HIPCHK(hipMemsetAsync(h.bpe, 0, sizeof(double) * (size_t)N, ctx->stream));
@@ -815,6 +1009,9 @@ int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const do
         }
     }
     s->masses.assign(masses, masses + h.ntypes);
+    for (int t = 0; t < kMaxTypes; ++t) s->type_count[t] = 0;
+    for (int k = 0; k < N; ++k) s->type_count[ty[k] - 1] += 1;
+    refresh_groups(s);
     for (int t = 0; t < h.ntypes; ++t) {
         if (!(masses[t] > 0.0)) return fail(ctx, GDPD_ERR_ARG, "mass of type %d must be positive", t + 1);
         h.mass[t] = masses[t];
@@ -1206,7 +1403,14 @@ static int nve_hook(gdpd_ctx *ctx, int sid, int initial)
     if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before using hooks");
     int rc = prepare(ctx, s);
     if (rc) return rc;
-    launch_nve(ctx->d_reps, s->tab.beads, s->tab.n_beads, initial, ctx->stream);
+    if (s->fix_path) {
+        // the nve fixes of the groups
+        const unsigned nve = fix_mask(s, GDPD_FIX_NVE);
+        if (initial) launch_fix_initial(ctx->d_reps, s->tab.beads, s->tab.n_beads, nve, ctx->stream);
+        else launch_fix_final(ctx->d_reps, s->tab.beads, s->tab.n_beads, nve, ctx->stream);
+    } else {
+        launch_nve(ctx->d_reps, s->tab.beads, s->tab.n_beads, initial, ctx->stream);
+    }
     if ((rc = check_launch(ctx))) return rc;
     if (initial) s->cells_valid = false;
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1253,8 +1457,14 @@ int gdpd_zero_step(gdpd_ctx *ctx, int sid)
 {
     Sys *s = get_sys(ctx, sid);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
-    int rc = ensure_cells(ctx, s);
-    if (rc) return rc;
+    int rc;
+    if (s->fix_path) {
+        if ((rc = prepare(ctx, s))) return rc;
+        if ((rc = fix_zero_step(ctx, s, true))) return rc;
+        s->midstep = true;
+        return GDPD_OK;
+    }
+    if ((rc = ensure_cells(ctx, s))) return rc;
     if ((rc = set_state_step(ctx, s, 0, true))) return rc;
     if ((rc = step_once(ctx, s->tab, kModeDrift))) return rc;
     if ((rc = maybe_resort(ctx, s, s->tab, 1))) return rc;
@@ -1268,6 +1478,13 @@ int gdpd_advance(gdpd_ctx *ctx, int sid, int nsteps)
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
     if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "call gdpd_zero_step first");
     if (nsteps <= 0) return GDPD_OK;
+    if (s->fix_path) {
+        for (int k = 0; k < nsteps; ++k) {
+            int rc = fix_iteration(ctx, s, true, true, true);
+            if (rc) return rc;
+        }
+        return GDPD_OK;
+    }
     const int gs = s->graph_steps > 0 ? s->graph_steps : 8;
     return advance_tables(ctx, s->tab, s->graph, nsteps, gs, {s});
 }
@@ -1277,7 +1494,8 @@ int gdpd_finish(gdpd_ctx *ctx, int sid)
     Sys *s = get_sys(ctx, sid);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
     if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "call gdpd_zero_step first");
-    int rc = step_once(ctx, s->tab, kModeFinalKick | kModeStoreForce);
+    int rc = s->fix_path ? fix_iteration(ctx, s, true, true, false)
+                         : step_once(ctx, s->tab, kModeFinalKick | kModeStoreForce);
     if (rc) return rc;
     s->midstep = false;
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1291,6 +1509,11 @@ int gdpd_run(gdpd_ctx *ctx, int sid, int nsteps)
     if (nsteps < 0) return fail(ctx, GDPD_ERR_ARG, "nsteps must be >= 0");
     int rc = ensure_cells(ctx, s);
     if (rc) return rc;
+    if (nsteps == 0 && s->fix_path) {
+        if ((rc = fix_zero_step(ctx, s, false))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return check_flags(ctx, s);
+    }
     if (nsteps == 0) {
         if ((rc = set_state_step(ctx, s, 0, true))) return rc;
         if ((rc = step_once(ctx, s->tab, kModeStoreForce))) return rc;
@@ -1335,6 +1558,243 @@ int gdpd_get_step(gdpd_ctx *ctx, int sid, int64_t *step)
     return GDPD_OK;
 }
 
+
+// ---- group fixes and computes --------------------------------------------------
+
+int gdpd_set_fixes(gdpd_ctx *ctx, int sid, const gdpd_fix *fixes, int n)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || n < 0 || (n > 0 && !fixes)) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before changing fixes");
+    RepDev &h = s->h;
+    const unsigned all = all_types(s);
+    std::vector<gdpd_fix> v(fixes, fixes + n);
+    bool has_nve = false;
+    unsigned integrated = 0;
+    for (int k = 0; k < n; ++k) {
+        const gdpd_fix &f = v[k];
+        if (f.kind < GDPD_FIX_NVE || f.kind > GDPD_FIX_RVCM) return fail(ctx, GDPD_ERR_ARG, "fix %d: unknown kind %d", k, f.kind);
+        if (f.group == 0 || (f.group & ~all)) return fail(ctx, GDPD_ERR_ARG, "fix %d: group names a type outside 1..%d", k, h.ntypes);
+        if (f.kind == GDPD_FIX_NVE || f.kind == GDPD_FIX_MOVE) {
+            // a bead integrated twice per step is rejected (the reference would apply both)
+            if (integrated & f.group) return fail(ctx, GDPD_ERR_ARG, "fix %d: a type is already integrated by another fix", k);
+            integrated |= f.group;
+            has_nve = has_nve || f.kind == GDPD_FIX_NVE;
+        }
+        if ((f.kind == GDPD_FIX_WALL_REFLECT || f.kind == GDPD_FIX_TWO_WALL) && (f.dir < 0 || f.dir > 2))
+            return fail(ctx, GDPD_ERR_ARG, "fix %d: wall direction must be x, y or z", k);
+        if (f.kind == GDPD_FIX_WALL_REFLECT) {
+            // fix_wall_reflect.cu:25-27, 47-66
+            if (!(f.p[0] < f.p[1])) return fail(ctx, GDPD_ERR_ARG, "fix %d: lo_wall_pos should lower than hi_wall_pos", k);
+            if (f.p[0] < h.lo[f.dir] || f.p[1] > h.lo[f.dir] + h.L[f.dir])
+                return fail(ctx, GDPD_ERR_ARG, "fix %d: Wall position is out of the box!", k);
+        }
+        if (f.kind == GDPD_FIX_TWO_WALL && (f.group2 == 0 || f.group3 == 0 || ((f.group2 | f.group3) & ~all)))
+            return fail(ctx, GDPD_ERR_ARG, "fix %d: two_wall needs two wall groups of known types", k);
+        if ((f.kind == GDPD_FIX_RECENTER || f.kind == GDPD_FIX_RVCM) && f.every <= 0)
+            return fail(ctx, GDPD_ERR_ARG, "fix %d: frequency must be positive", k);
+        if (f.kind == GDPD_FIX_SPRING && f.every < 0) return fail(ctx, GDPD_ERR_ARG, "fix %d: file frequency must be >= 0", k);
+        if (f.kind == GDPD_FIX_SPRING || f.kind == GDPD_FIX_RECENTER) {
+            // fix_spring.cu:110-125, fix_recenter.cu:75-90
+            const double *c = f.kind == GDPD_FIX_SPRING ? f.p + 1 : f.p;
+            for (int d = 0; d < 3; ++d)
+                if (c[d] < h.lo[d] || c[d] > h.lo[d] + h.L[d])
+                    return fail(ctx, GDPD_ERR_ARG, "fix %d: position is out of the box in %c direction!", k, "xyz"[d]);
+        }
+    }
+    (void)has_nve;
+    if ((int)v.size() > kMaxFixes) return fail(ctx, GDPD_ERR_ARG, "at most %d fixes", kMaxFixes);
+    s->fixes = v;
+    finish_fix_setup(s);
+    return push_desc(ctx, sid);
+}
+
+int gdpd_set_computes(gdpd_ctx *ctx, int sid, const gdpd_compute *c, int n)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || n < 0 || (n > 0 && !c)) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before changing computes");
+    if (n > kMaxComputes) return fail(ctx, GDPD_ERR_ARG, "at most %d computes", kMaxComputes);
+    RepDev &h = s->h;
+    const unsigned all = all_types(s);
+    bool msd = false;
+    for (int k = 0; k < n; ++k) {
+        const gdpd_compute &q = c[k];
+        if (q.kind < GDPD_COMPUTE_TEMP || q.kind > GDPD_COMPUTE_WALL_CONTACT)
+            return fail(ctx, GDPD_ERR_ARG, "compute %d: unknown kind %d", k, q.kind);
+        if (q.group == 0 || (q.group & ~all)) return fail(ctx, GDPD_ERR_ARG, "compute %d: group names an unknown type", k);
+        if (q.every <= 0) return fail(ctx, GDPD_ERR_ARG, "compute %d: frequency must be positive", k);
+        if (q.kind == GDPD_COMPUTE_RHO) {
+            // compute_rho.cu:55-85
+            for (int d = 0; d < 3; ++d) {
+                if (q.p[2 * d] < h.lo[d]) return fail(ctx, GDPD_ERR_ARG, "compute %d: %clo < box.%clo", k, "xyz"[d], "xyz"[d]);
+                if (q.p[2 * d + 1] > h.lo[d] + h.L[d])
+                    return fail(ctx, GDPD_ERR_ARG, "compute %d: %chi > box.%chi", k, "xyz"[d], "xyz"[d]);
+                if (!(q.p[2 * d] < q.p[2 * d + 1])) return fail(ctx, GDPD_ERR_ARG, "compute %d: empty region", k);
+            }
+        }
+        if (q.kind == GDPD_COMPUTE_WALL_CONTACT &&
+            (q.group2 == 0 || (q.group2 & ~all) || (q.side != 1 && q.side != -1) || q.dir < 0 || q.dir > 2 || !(q.p[0] > 0)))
+            return fail(ctx, GDPD_ERR_ARG, "compute %d: wall_contact needs a wall group, lo|hi, x|y|z and a positive cutoff", k);
+        msd = msd || q.kind == GDPD_COMPUTE_MSD;
+    }
+    if (msd) {
+        // reference positions: the unwrapped state now (compute_msd.cu:40-50), by id
+        if (!s->uploaded) return fail(ctx, GDPD_ERR_STATE, "upload the state before setting an msd compute");
+        const int N = h.N;
+        std::vector<double> pos(3 * (size_t)N), x0(3 * (size_t)N);
+        std::vector<int> img(3 * (size_t)N);
+        int rc = gdpd_download(ctx, sid, pos.data(), nullptr, img.data(), nullptr);
+        if (rc) return rc;
+        for (int i = 0; i < N; ++i)
+            for (int d = 0; d < 3; ++d)
+                x0[(size_t)d * N + i] = pos[3 * (size_t)i + d] + img[3 * (size_t)i + d] * h.L[d];
+        if (!h.x0uw && (rc = dalloc(ctx, s, &h.x0uw, 3 * (size_t)N))) return rc;
+        HIPCHK(hipMemcpy(h.x0uw, x0.data(), sizeof(double) * x0.size(), hipMemcpyHostToDevice));
+    }
+    s->computes.assign(c, c + n);
+    h.ncomp = n;
+    for (int k = 0; k < n; ++k) {
+        CompDev &C = h.comp[k];
+        C = CompDev{};
+        C.kind = c[k].kind;
+        C.group = c[k].group;
+        C.group2 = c[k].group2;
+        C.every = c[k].every;
+        C.dir = c[k].dir;
+        C.side = c[k].side;
+        for (int q = 0; q < 6; ++q) C.p[q] = c[k].p[q];
+    }
+    finish_fix_setup(s);
+    return push_desc(ctx, sid);
+}
+
+// newest record of a slot, copied out after a sync
+static int last_record(gdpd_ctx *ctx, Sys *s, int slot, double *out)
+{
+    int nrec = 0;
+    HIPCHK(hipMemcpy(&nrec, s->h.rec_n + slot, sizeof(int), hipMemcpyDeviceToHost));
+    if (nrec <= 0) return fail(ctx, GDPD_ERR_STATE, "no record");
+    const int row = (nrec - 1) % kRecCap;
+    HIPCHK(hipMemcpy(out, s->h.rec_rows + ((size_t)slot * kRecCap + row) * kRecLen, sizeof(double) * kRecLen,
+                     hipMemcpyDeviceToHost));
+    return GDPD_OK;
+}
+
+int gdpd_fix_hook(gdpd_ctx *ctx, int sid, int index, int hook, uint32_t step, double *out5)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before using hooks");
+    if (index < 0 || index >= s->h.nfix) return fail(ctx, GDPD_ERR_ARG, "no fix %d", index);
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    if ((rc = check_groups(ctx, s))) return rc;
+    Tables &T = s->tab;
+    hipStream_t st = ctx->stream;
+    const FixDev &F = s->h.fix[index];
+    const unsigned one = 1u << index;
+    const bool fire = F.every > 0 && (step % (uint32_t)F.every) == 0;
+    bool moved = false;
+    switch (hook) {
+    case GDPD_HOOK_INITIAL_INTEGRATE:
+        if (F.kind == GDPD_FIX_NVE || F.kind == GDPD_FIX_MOVE) {
+            launch_fix_initial(ctx->d_reps, T.beads, T.n_beads, one, st);
+            moved = true;
+        }
+        break;
+    case GDPD_HOOK_POST_INTEGRATE:
+        if (F.kind == GDPD_FIX_WALL_REFLECT) {
+            launch_fix_initial(ctx->d_reps, T.beads, T.n_beads, one, st);
+            moved = true;
+        } else if (F.kind == GDPD_FIX_TWO_WALL) {
+            launch_fix_clear(ctx->d_reps, T.rep_list, T.n_reps, st);
+            launch_fix_initial(ctx->d_reps, T.beads, T.n_beads, one, st);
+            launch_fix_two_wall(ctx->d_reps, T.beads, T.n_beads, one, st);
+            moved = true;
+        }
+        break;
+    case GDPD_HOOK_POST_FORCE:
+        if (F.kind == GDPD_FIX_SPRING) {
+            launch_fix_clear(ctx->d_reps, T.rep_list, T.n_reps, st);
+            launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, one, 0u, 0, st);
+            launch_fix_post_force(ctx->d_reps, T.beads, T.n_beads, one, st);
+            launch_fix_records(ctx->d_reps, T.rep_list, T.n_reps, one, 0u, (long long)step, st);
+        }
+        break;
+    case GDPD_HOOK_FINAL_INTEGRATE:
+        if (F.kind == GDPD_FIX_NVE) launch_fix_final(ctx->d_reps, T.beads, T.n_beads, one, st);
+        break;
+    case GDPD_HOOK_END_OF_STEP:
+        if ((F.kind == GDPD_FIX_RVCM || F.kind == GDPD_FIX_RECENTER) && fire) {
+            launch_fix_clear(ctx->d_reps, T.rep_list, T.n_reps, st);
+            launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, one, 0u, F.kind == GDPD_FIX_RVCM ? 1 : 0, st);
+            launch_fix_end(ctx->d_reps, T.beads, T.n_beads, one, st);
+            moved = F.kind == GDPD_FIX_RECENTER;
+        }
+        break;
+    default:
+        return fail(ctx, GDPD_ERR_ARG, "unknown hook %d", hook);
+    }
+    if ((rc = check_launch(ctx))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (moved) s->cells_valid = false;
+    if (out5 && hook == GDPD_HOOK_POST_FORCE && F.kind == GDPD_FIX_SPRING) {
+        double r[kRecLen];
+        if ((rc = last_record(ctx, s, kMaxComputes + index, r))) return rc;
+        for (int q = 0; q < 5; ++q) out5[q] = r[1 + q];
+    }
+    return check_flags(ctx, s);
+}
+
+int gdpd_compute_eval(gdpd_ctx *ctx, int sid, int index, double *out3)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !out3) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (index < 0 || index >= s->h.ncomp) return fail(ctx, GDPD_ERR_ARG, "no compute %d", index);
+    int rc = prepare(ctx, s);
+    if (rc) return rc;
+    if ((rc = check_groups(ctx, s))) return rc;
+    Tables &T = s->tab;
+    hipStream_t st = ctx->stream;
+    const unsigned one = 1u << index;
+    const int kind = s->h.comp[index].kind;
+    launch_fix_clear(ctx->d_reps, T.rep_list, T.n_reps, st);
+    if (kind == GDPD_COMPUTE_TF) {
+        launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, 0u, one, 1, st);
+    } else {
+        launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, 0u, one, 2, st);
+        if (kind == GDPD_COMPUTE_WALL_CONTACT) launch_fix_reduce(ctx->d_reps, T.beads, T.n_beads, 0u, one, 3, st);
+    }
+    launch_fix_records(ctx->d_reps, T.rep_list, T.n_reps, 0u, one, -1, st);
+    if ((rc = check_launch(ctx))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    double r[kRecLen];
+    if ((rc = last_record(ctx, s, index, r))) return rc;
+    for (int q = 0; q < 3; ++q) out3[q] = r[1 + q];
+    return GDPD_OK;
+}
+
+int gdpd_records(gdpd_ctx *ctx, int sid, int index, double *out, int max_rows, int *n_rows)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !n_rows || index < 0 || index >= kRecSlots) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int nrec = 0;
+    HIPCHK(hipMemcpy(&nrec, s->h.rec_n + index, sizeof(int), hipMemcpyDeviceToHost));
+    const int have = std::min(nrec, kRecCap);
+    const int first = nrec - have;
+    std::vector<double> rows((size_t)kRecCap * kRecLen);
+    HIPCHK(hipMemcpy(rows.data(), s->h.rec_rows + (size_t)index * kRecCap * kRecLen, sizeof(double) * rows.size(),
+                     hipMemcpyDeviceToHost));
+    const int n = std::min(have, std::max(0, max_rows));
+    for (int k = 0; k < n && out; ++k)
+        memcpy(out + (size_t)k * kRecLen, rows.data() + (size_t)((first + k) % kRecCap) * kRecLen,
+               sizeof(double) * kRecLen);
+    *n_rows = n;
+    return GDPD_OK;
+}
+
 int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n, int nsteps)
 {
     if (!ctx || !systems || n <= 0) return fail(ctx, GDPD_ERR_ARG, "bad batch arguments");
@@ -1345,6 +1805,7 @@ int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n, int nsteps)
         Sys *s = get_sys(ctx, id);
         if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", id);
         if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "system %d: call gdpd_zero_step first", id);
+        if (s->fix_path) return fail(ctx, GDPD_ERR_ARG, "system %d has group fixes or computes: advance it alone", id);
         if (prec >= 0 && s->h.precision != prec) return fail(ctx, GDPD_ERR_ARG, "a batch must share one precision");
         prec = s->h.precision;
         members.push_back(s);

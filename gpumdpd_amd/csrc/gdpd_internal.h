@@ -14,11 +14,20 @@ constexpr int kThermoParts = 10;   // ke, pair pe, bond pe, p[6], angle pe
 constexpr int kThermoRowLen = 13;  // one device thermo record (GDPD_THERMO_LEN)
 constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
 
+// group fixes and computes (GDPD_MAX_FIXES, GDPD_MAX_COMPUTES, GDPD_RECORD_*)
+constexpr int kMaxFixes = 8;
+constexpr int kMaxComputes = 8;
+constexpr int kRecLen = 6;                         // {step, up to 5 values}
+constexpr int kRecCap = 1024;                      // ring of records per slot
+constexpr int kRecSlots = kMaxComputes + kMaxFixes; // compute k -> k, fix k -> kMaxComputes + k
+constexpr int kAccPer = 4;                         // reduced quantities per slot
+
 // device flag bits (RepState::flags)
 constexpr int kFlagStageOverflow = 1;
 constexpr int kFlagListOverflow = 2;
 constexpr int kFlagNonFinite = 4;
 constexpr int kFlagNeighOverflow = 8;
+constexpr int kFlagWallOrder = 16;   // two_wall: the upper wall is not above the lower wall
 
 // step-kernel mode bits
 constexpr int kModeFinalKick = 1;  // v += dt/2 F/m before thermo (not at step 0)
@@ -43,6 +52,24 @@ struct RepState {
     int flags;        // kFlag* bits, sticky
     int log_rows;     // thermo records written in this run
     int pad;
+};
+
+// One fix (gdpd_fix) with its group's bead count and mass, set on the host.
+struct FixDev {
+    int kind;
+    unsigned group, group2, group3;  // type masks, bit t-1 for type t
+    int every, dir, flags, gcount;
+    double p[5];
+    double gmass;
+};
+
+// One compute (gdpd_compute) with its group's bead count and mass.
+struct CompDev {
+    int kind;
+    unsigned group, group2;
+    int every, dir, side, gcount, pad;
+    double p[6];
+    double gmass;
 };
 
 // One replica as the kernels see it.  All bead arrays are in "slot" order; the
@@ -89,6 +116,15 @@ struct RepDev {
     double *thermo_bead;       // [ceil(N/256)][kThermoParts]: KE, bond terms, m v v
     double *thermo_rows;       // [thermo_cap][kThermoRowLen]
     int thermo_cap;
+    // group fixes and computes, in the order given
+    int nfix, ncomp;
+    FixDev fix[kMaxFixes];
+    CompDev comp[kMaxComputes];
+    long long *racc;           // [kRecSlots][kAccPer][2]: two-word fixed-point sums
+    unsigned long long *rext;  // [kRecSlots][2]: order-preserving keys of a max and a min
+    double *rec_rows;          // [kRecSlots][kRecCap][kRecLen]
+    int *rec_n;                // [kRecSlots]: records written
+    double *x0uw;              // [3N] by id-1: msd reference (unwrapped) positions
     RepState *state;
     // scratch for re-sorting
     double *scratch;           // 3N doubles

@@ -33,7 +33,18 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
 void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, hipStream_t s);
 // per replica (rep_list: nreps replica indices)
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
-                     int thermo, hipStream_t s);
+                     int thermo, hipStream_t s, int bricks = 1);
+// group fixes and computes: fmask / cmask select fixes / computes by index
+void launch_fix_clear(const RepDev *reps, const int *rep_list, int nreps, hipStream_t s);
+void launch_fix_initial(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s);
+void launch_fix_two_wall(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s);
+void launch_fix_reduce(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, unsigned cmask,
+                       int phase, hipStream_t s);
+void launch_fix_post_force(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s);
+void launch_fix_final(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s);
+void launch_fix_end(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s);
+void launch_fix_records(const RepDev *reps, const int *rep_list, int nreps, unsigned fmask, unsigned cmask,
+                        long long step, hipStream_t s);
 // hook path
 void launch_nve(const RepDev *reps, const WorkItem *items, int nitems, int initial, hipStream_t s);
 void launch_bond_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s);

@@ -1225,7 +1225,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
 
 __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ reps,
                                                      const int *__restrict__ rep_list, int flip,
-                                                     int advance, int thermo)
+                                                     int advance, int thermo, int bricks)
 {
     __shared__ long long red[64];
     __shared__ long long itot[kThermoParts];
@@ -1237,7 +1237,7 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
 #pragma unroll
         for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
         const long long *tp = (const long long *)R.thermo_part, *tb = (const long long *)R.thermo_bead;
-        for (int b = threadIdx.x; b < R.nbrick; b += kBlock)
+        for (int b = threadIdx.x; bricks && b < R.nbrick; b += kBlock)
 #pragma unroll
             for (int k = 0; k < kThermoParts; ++k) acc[k] += tp[(size_t)b * kThermoParts + k];
         const int nchunk = (R.N + kBlock - 1) / kBlock;
@@ -1279,6 +1279,486 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
     if (threadIdx.x == 0) {
         R.state->step = step + advance;
         R.state->cur ^= flip;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// group fixes and computes (fix/fix_*.cu, compute/compute_*.cu).  A group is a
+// mask of bead types, so membership is one bit test per bead and no index list
+// is kept.  Every reduction is two-word fixed point: v * 2^16 split into its
+// integer part and the remaining fraction * 2^32, each summed as int64.  The
+// sums are exact and commute, so results do not depend on bead order, block
+// count or slot layout; resolution is 2^-48 per term over |v| < 2^40 / N.
+// Extents (wall positions) use order-preserving integer keys of the doubles.
+// ---------------------------------------------------------------------------
+
+constexpr double kSplitHi = 65536.0, kSplitLo = 4294967296.0;
+
+__device__ __forceinline__ void split2(double v, long long &h, long long &l)
+{
+    const double t = v * kSplitHi, f = floor(t);
+    h = (long long)f;
+    l = __double2ll_rn((t - f) * kSplitLo);
+}
+
+__host__ __device__ __forceinline__ double join2(const long long *a)
+{
+    return (double)a[0] * (1.0 / kSplitHi) + (double)a[1] * (1.0 / (kSplitHi * kSplitLo));
+}
+
+__device__ __forceinline__ unsigned long long order_key(double d)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__host__ __device__ __forceinline__ double order_key_value(unsigned long long k)
+{
+    const unsigned long long u = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    double d;
+    memcpy(&d, &u, sizeof(d));
+    return d;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Block-level accumulators of one fix/compute kernel: per-wave exact sums
+// land in LDS, then one global atomic per nonzero word per block.
+struct FixBlockAcc {
+    long long *sum;            // [kRecSlots][kAccPer][2]
+    unsigned long long *ext;   // [kRecSlots][2]
+};
+
+__device__ __forceinline__ void fba_init(FixBlockAcc &A)
+{
+    for (int i = threadIdx.x; i < kRecSlots * kAccPer * 2; i += kBlock) A.sum[i] = 0;
+    for (int i = threadIdx.x; i < kRecSlots; i += kBlock) {
+        A.ext[2 * i] = 0ull;
+        A.ext[2 * i + 1] = ~0ull;
+    }
+    __syncthreads();
+}
+
+// every lane of the wave calls it; inactive lanes add zero
+__device__ __forceinline__ void fba_add(FixBlockAcc &A, int slot, int q, double v, bool active)
+{
+    long long h = 0, l = 0;
+    if (active) split2(v, h, l);
+    h = wave_sum_i64(h);
+    l = wave_sum_i64(l);
+    if ((threadIdx.x & 63) == 0 && (h | l)) {
+        atomicAdd((unsigned long long *)&A.sum[(slot * kAccPer + q) * 2], (unsigned long long)h);
+        atomicAdd((unsigned long long *)&A.sum[(slot * kAccPer + q) * 2 + 1], (unsigned long long)l);
+    }
+}
+
+__device__ __forceinline__ void fba_max(FixBlockAcc &A, int slot, double v, bool active)
+{
+    const unsigned long long k = wave_max_u64(active ? order_key(v) : 0ull);
+    if ((threadIdx.x & 63) == 0 && k) atomicMax(&A.ext[2 * slot], k);
+}
+
+__device__ __forceinline__ void fba_min(FixBlockAcc &A, int slot, double v, bool active)
+{
+    const unsigned long long k = wave_min_u64(active ? order_key(v) : ~0ull);
+    if ((threadIdx.x & 63) == 0 && ~k) atomicMin(&A.ext[2 * slot + 1], k);
+}
+
+__device__ __forceinline__ void fba_flush(FixBlockAcc &A, const RepDev &R)
+{
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRecSlots * kAccPer * 2; i += kBlock)
+        if (A.sum[i]) atomicAdd((unsigned long long *)&R.racc[i], (unsigned long long)A.sum[i]);
+    for (int i = threadIdx.x; i < kRecSlots; i += kBlock) {
+        if (A.ext[2 * i]) atomicMax(&R.rext[2 * i], A.ext[2 * i]);
+        if (~A.ext[2 * i + 1]) atomicMin(&R.rext[2 * i + 1], A.ext[2 * i + 1]);
+    }
+}
+
+__device__ __forceinline__ double racc_value(const RepDev &R, int slot, int q)
+{
+    return join2(&R.racc[(slot * kAccPer + q) * 2]);
+}
+
+__device__ __forceinline__ bool in_group(unsigned mask, int type) { return (mask >> (type - 1)) & 1u; }
+
+// reflection about a plane (fix/fix_wall_reflect.cu:96-118, fix_two_wall.cu:118-140):
+// below lo, or else above hi, the coordinate is mirrored and the velocity flipped
+__device__ __forceinline__ void reflect_axis(double &x, double &v, double lo, double hi)
+{
+    if (x < lo) {
+        x += 2.0 * (lo - x);
+        v = -v;
+    } else if (x > hi) {
+        x += 2.0 * (hi - x);
+        v = -v;
+    }
+}
+
+// spring of fix/fix_spring.cu:300-350 on the group's centre of mass (from the
+// position sums of slot): total force {fx,fy,fz}, signed magnitude, energy
+__device__ __forceinline__ void spring_terms(const RepDev &R, const FixDev &F, int slot, double (&o)[5])
+{
+    double d[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d[a] = racc_value(R, slot, a) / F.gmass - F.p[1 + a];
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double dr = r - F.p[4];
+    o[0] = o[1] = o[2] = o[3] = o[4] = 0.0;
+    if (r > 1.0e-10) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) o[a] = -F.p[0] * dr * d[a] / r;
+        o[3] = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+        if (dr < 0.0) o[3] = -o[3];
+        o[4] = 0.5 * F.p[0] * dr * dr;
+    }
+}
+
+// zero the sums and reset the extents of every slot (one block per replica)
+__global__ __launch_bounds__(kBlock) void k_fix_clear(const RepDev *__restrict__ reps, const int *__restrict__ rep_list)
+{
+    const RepDev &R = reps[rep_list[blockIdx.x]];
+    for (int i = threadIdx.x; i < kRecSlots * kAccPer * 2; i += kBlock) R.racc[i] = 0;
+    for (int i = threadIdx.x; i < kRecSlots; i += kBlock) {
+        R.rext[2 * i] = 0ull;
+        R.rext[2 * i + 1] = ~0ull;
+    }
+}
+
+// initial_integrate and post_integrate of the fixes in fmask, in order:
+// nve (fix_nve.cu:30-77: kick, drift, wrap), move (fix_move.cu:45-90: velocity
+// set, drift, wrap), then wall_reflect; then the wall extents of two_wall
+// (its reflection needs the whole group, k_fix_two_wall)
+__global__ __launch_bounds__(kBlock) void k_fix_initial(const RepDev *__restrict__ reps,
+                                                        const WorkItem *__restrict__ items, int nitems,
+                                                        unsigned fmask)
+{
+    __shared__ long long lsum[kRecSlots * kAccPer * 2];
+    __shared__ unsigned long long lext[kRecSlots * 2];
+    FixBlockAcc A{lsum, lext};
+    fba_init(A);
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    const bool live = s < R.N;
+    const int N = R.N, cur = R.state->cur;
+    double *X = R.x[cur], *V = R.v[cur];
+    const int type = live ? (int)R.type[s] : 1;
+    double x[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    int im[3] = {0, 0, 0};
+    bool moved = false, bad = false;
+    if (live) {
+        unpack_img(R.img[s], im[0], im[1], im[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x[d] = X[d * N + s];
+            v[d] = V[d * N + s];
+        }
+    }
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (!live || !in_group(F.group, type)) continue;
+        if (F.kind == 1) {  // nve
+            const double h = (0.5 * R.dt) * R.minv[type - 1];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                v[d] = fma(h, R.force[d * N + s], v[d]);
+                x[d] = wrap_axis(fma(R.dt, v[d], x[d]), R.lo[d], R.lo[d] + R.L[d], R.L[d], im[d], bad);
+            }
+            moved = true;
+        } else if (F.kind == 2) {  // move
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                v[d] = F.p[d];
+                x[d] = wrap_axis(x[d] + F.p[d] * R.dt, R.lo[d], R.lo[d] + R.L[d], R.L[d], im[d], bad);
+            }
+            moved = true;
+        }
+    }
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (F.kind != 3 || !live || !in_group(F.group, type)) continue;
+        const int d = F.dir;
+        const double x0 = x[d], v0 = v[d];
+        reflect_axis(x[d], v[d], F.p[0], F.p[1]);
+        moved = moved || x[d] != x0 || v[d] != v0;
+    }
+    if (moved) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            X[d * N + s] = x[d];
+            V[d * N + s] = v[d];
+        }
+        R.img[s] = pack_img(im[0], im[1], im[2]);
+        if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
+    }
+    bool any_ext = false;
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (F.kind != 4) continue;
+        any_ext = true;
+        fba_max(A, kMaxComputes + k, x[F.dir], live && in_group(F.group2, type));
+        fba_min(A, kMaxComputes + k, x[F.dir], live && in_group(F.group3, type));
+    }
+    if (any_ext) fba_flush(A, R);
+}
+
+// two_wall reflection of the group between the top of the lower wall and the
+// bottom of the upper wall (fix/fix_two_wall.cu:145-204)
+__global__ __launch_bounds__(kBlock) void k_fix_two_wall(const RepDev *__restrict__ reps,
+                                                         const WorkItem *__restrict__ items, int nitems,
+                                                         unsigned fmask)
+{
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    if (s >= R.N) return;
+    const int N = R.N, cur = R.state->cur, type = R.type[s];
+    double *X = R.x[cur], *V = R.v[cur];
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (F.kind != 4) continue;
+        const int slot = kMaxComputes + k;
+        const double lo = order_key_value(R.rext[2 * slot]), hi = order_key_value(R.rext[2 * slot + 1]);
+        if (s == 0 && !(hi > lo)) atomicOr(&R.state->flags, kFlagWallOrder);
+        if (!in_group(F.group, type)) continue;
+        const int d = F.dir;
+        double x = X[d * N + s], v = V[d * N + s];
+        const double x0 = x;
+        reflect_axis(x, v, lo, hi);
+        if (x != x0) {
+            X[d * N + s] = x;
+            V[d * N + s] = v;
+        }
+    }
+}
+
+// Group sums of one phase for the fixes in fmask and the computes in cmask.
+//   phase 0: spring / recenter: sum m x_unwrapped (fix_spring.cu:240-290)
+//   phase 1: rvcm: sum m v (fix_rvcm.cu:95-150); tf: sum f (compute_tf.cu:60-110)
+//   phase 2: temp: sum m v^2 / 2 (compute_temp.cu:60-110); msd: sum |x_uw - x0|^2
+//            (compute_msd.cu:70-120); rho: count in the region (compute_rho.cu:100-160);
+//            wall_contact: extreme of the wall group (compute_wall_contact.cu:200-270)
+//   phase 3: wall_contact: count in the contact slab (compute_wall_contact.cu:120-170)
+__global__ __launch_bounds__(kBlock) void k_fix_reduce(const RepDev *__restrict__ reps,
+                                                       const WorkItem *__restrict__ items, int nitems,
+                                                       unsigned fmask, unsigned cmask, int phase)
+{
+    __shared__ long long lsum[kRecSlots * kAccPer * 2];
+    __shared__ unsigned long long lext[kRecSlots * 2];
+    FixBlockAcc A{lsum, lext};
+    fba_init(A);
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    const bool live = s < R.N;
+    const int N = R.N, cur = R.state->cur;
+    const int sv = live ? s : 0;
+    const int type = R.type[sv];
+    const double m = R.mass[type - 1];
+    const double *X = R.x[cur], *V = R.v[cur];
+    int im[3];
+    unpack_img(R.img[sv], im[0], im[1], im[2]);
+    double x[3], xu[3], v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = X[d * N + sv];
+        xu[d] = x[d] + (double)im[d] * R.L[d];
+        v[d] = V[d * N + sv];
+    }
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        const bool g = live && in_group(F.group, type);
+        const int slot = kMaxComputes + k;
+        if (phase == 0 && (F.kind == 5 || F.kind == 6)) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fba_add(A, slot, d, m * xu[d], g);
+        } else if (phase == 1 && F.kind == 7) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fba_add(A, slot, d, m * v[d], g);
+        }
+    }
+    for (int k = 0; k < R.ncomp; ++k) {
+        if (!((cmask >> k) & 1u)) continue;
+        const CompDev &C = R.comp[k];
+        const bool g = live && in_group(C.group, type);
+        if (phase == 1 && C.kind == 4) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) fba_add(A, k, d, R.force[d * N + sv], g);
+        } else if (phase == 2 && C.kind == 1) {
+            fba_add(A, k, 0, 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), g);
+        } else if (phase == 2 && C.kind == 2) {
+            const int i = R.id[sv] - 1;
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double dd = xu[d] - R.x0uw[(size_t)d * N + i];
+                r2 += dd * dd;
+            }
+            fba_add(A, k, 0, r2, g);
+        } else if (phase == 2 && C.kind == 3) {
+            const bool in = x[0] >= C.p[0] && x[0] < C.p[1] && x[1] >= C.p[2] && x[1] < C.p[3] && x[2] >= C.p[4] &&
+                            x[2] < C.p[5];
+            fba_add(A, k, 0, 1.0, g && in);
+        } else if (phase == 2 && C.kind == 5) {
+            const bool w = live && in_group(C.group2, type);
+            if (C.side > 0) fba_max(A, k, x[C.dir], w);
+            else fba_min(A, k, x[C.dir], w);
+        } else if (phase == 3 && C.kind == 5) {
+            const double xd = x[C.dir];
+            bool in;
+            if (C.side > 0) {
+                const double b = order_key_value(R.rext[2 * k]);
+                in = xd >= b && xd < b + C.p[0];
+            } else {
+                const double b = order_key_value(R.rext[2 * k + 1]);
+                in = xd >= b - C.p[0] && xd < b;
+            }
+            fba_add(A, k, 0, 1.0, g && in);
+        }
+    }
+    fba_flush(A, R);
+}
+
+// post_force of spring (fix/fix_spring.cu:300-376): the group's spring force,
+// shared in proportion to mass, added to the force
+__global__ __launch_bounds__(kBlock) void k_fix_post_force(const RepDev *__restrict__ reps,
+                                                           const WorkItem *__restrict__ items, int nitems,
+                                                           unsigned fmask)
+{
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    if (s >= R.N) return;
+    const int N = R.N, type = R.type[s];
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (F.kind != 5 || !in_group(F.group, type)) continue;
+        double o[5];
+        spring_terms(R, F, kMaxComputes + k, o);
+        const double m = R.mass[type - 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) R.force[d * N + s] += (o[d] / F.gmass) * m;
+    }
+}
+
+// final_integrate of nve (fix/fix_nve.cu:120-152)
+__global__ __launch_bounds__(kBlock) void k_fix_final(const RepDev *__restrict__ reps,
+                                                      const WorkItem *__restrict__ items, int nitems,
+                                                      unsigned fmask)
+{
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    if (s >= R.N) return;
+    const int N = R.N, type = R.type[s];
+    double *V = R.v[R.state->cur];
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (F.kind != 1 || !in_group(F.group, type)) continue;
+        const double h = (0.5 * R.dt) * R.minv[type - 1];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) V[d * N + s] = fma(h, R.force[d * N + s], V[d * N + s]);
+    }
+}
+
+// end_of_step of rvcm (fix/fix_rvcm.cu:220-292: v -= flagged components of the
+// group's centre-of-mass velocity) and recenter (fix/fix_recenter.cu:260-340:
+// the group is translated so its unwrapped centre of mass sits at the target)
+__global__ __launch_bounds__(kBlock) void k_fix_end(const RepDev *__restrict__ reps,
+                                                    const WorkItem *__restrict__ items, int nitems,
+                                                    unsigned fmask)
+{
+    const WorkItem it = items[xcd_item(nitems)];
+    const RepDev &R = reps[it.rep];
+    const int s = it.local * kBlock + threadIdx.x;
+    if (s >= R.N) return;
+    const int N = R.N, type = R.type[s], cur = R.state->cur;
+    double *X = R.x[cur], *V = R.v[cur];
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (!in_group(F.group, type)) continue;
+        const int slot = kMaxComputes + k;
+        if (F.kind == 7) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if ((F.flags >> d) & 1) V[d * N + s] -= racc_value(R, slot, d) / F.gmass;
+        } else if (F.kind == 6) {
+            int im[3];
+            unpack_img(R.img[s], im[0], im[1], im[2]);
+            bool bad = false;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double shift = F.p[d] - racc_value(R, slot, d) / F.gmass;
+                X[d * N + s] = wrap_axis(X[d * N + s] + shift, R.lo[d], R.lo[d] + R.L[d], R.L[d], im[d], bad);
+            }
+            R.img[s] = pack_img(im[0], im[1], im[2]);
+            if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
+        }
+    }
+}
+
+// one record per firing compute and per recording spring (one block per replica)
+__global__ __launch_bounds__(64) void k_fix_records(const RepDev *__restrict__ reps, const int *__restrict__ rep_list,
+                                                    unsigned fmask, unsigned cmask, long long step)
+{
+    const RepDev &R = reps[rep_list[blockIdx.x]];
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < R.ncomp; ++k) {
+        if (!((cmask >> k) & 1u)) continue;
+        const CompDev &C = R.comp[k];
+        double o[5] = {0, 0, 0, 0, 0};
+        if (C.kind == 1) o[0] = 2.0 * racc_value(R, k, 0) / (3.0 * C.gcount - 3.0);
+        else if (C.kind == 2) o[0] = racc_value(R, k, 0) / (double)C.gcount;
+        else if (C.kind == 3)
+            o[0] = racc_value(R, k, 0) / ((C.p[1] - C.p[0]) * (C.p[3] - C.p[2]) * (C.p[5] - C.p[4]));
+        else if (C.kind == 4)
+            for (int d = 0; d < 3; ++d) o[d] = racc_value(R, k, d);
+        else if (C.kind == 5) o[0] = racc_value(R, k, 0);
+        const int row = R.rec_n[k] % kRecCap;
+        double *r = R.rec_rows + ((size_t)k * kRecCap + row) * kRecLen;
+        r[0] = (double)step;
+        for (int q = 0; q < 5; ++q) r[1 + q] = o[q];
+        R.rec_n[k] += 1;
+    }
+    for (int k = 0; k < R.nfix; ++k) {
+        if (!((fmask >> k) & 1u)) continue;
+        const FixDev &F = R.fix[k];
+        if (F.kind != 5) continue;
+        const int slot = kMaxComputes + k;
+        double o[5];
+        spring_terms(R, F, slot, o);
+        const int row = R.rec_n[slot] % kRecCap;
+        double *r = R.rec_rows + ((size_t)slot * kRecCap + row) * kRecLen;
+        r[0] = (double)step;
+        for (int q = 0; q < 5; ++q) r[1 + q] = o[q];
+        R.rec_n[slot] += 1;
     }
 }
 
@@ -1384,12 +1864,16 @@ __global__ __launch_bounds__(kBlock) void k_angle_hook(const RepDev *__restrict_
     }
 }
 
+// per-chunk thermo partials from the per-bead arrays: FP64 into partials, or
+// (partials == nullptr) fixed point into the replica's thermo_bead like the
+// fused integrate writes them
 __global__ __launch_bounds__(kBlock) void k_thermo_arrays(const RepDev *__restrict__ reps,
                                                           const WorkItem *__restrict__ items,
                                                           double *__restrict__ partials)
 {
     __shared__ double red[16 * (kBlock / 64)];
     __shared__ double out[kThermoParts];
+    __shared__ long long fred[64];
     const WorkItem it = items[blockIdx.x];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
@@ -1411,6 +1895,13 @@ __global__ __launch_bounds__(kBlock) void k_thermo_arrays(const RepDev *__restri
         acc[6] = m * vy * vy + R.vir[3 * N + s];
         acc[7] = m * vy * vz + R.vir[4 * N + s];
         acc[8] = m * vz * vz + R.vir[5 * N + s];
+    }
+    if (!partials) {
+        long long *fout = fred + 48;
+        block_sum_fix<kThermoParts>(acc, fred, fout);
+        if (threadIdx.x < kThermoParts)
+            ((long long *)R.thermo_bead)[(size_t)it.local * kThermoParts + threadIdx.x] = fout[threadIdx.x];
+        return;
     }
     block_sum<kThermoParts>(acc, red, out);
     if (threadIdx.x < kThermoParts) partials[(size_t)blockIdx.x * kThermoParts + threadIdx.x] = out[threadIdx.x];
@@ -1634,9 +2125,45 @@ void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int
     if (nitems) hipLaunchKernelGGL(k_integrate, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, mode);
 }
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
-                     int thermo, hipStream_t s)
+                     int thermo, hipStream_t s, int bricks)
 {
-    if (nreps) hipLaunchKernelGGL(k_step_end, dim3(nreps), dim3(kBlock), 0, s, reps, rep_list, flip, advance, thermo);
+    if (nreps)
+        hipLaunchKernelGGL(k_step_end, dim3(nreps), dim3(kBlock), 0, s, reps, rep_list, flip, advance, thermo, bricks);
+}
+void launch_fix_clear(const RepDev *reps, const int *rep_list, int nreps, hipStream_t s)
+{
+    if (nreps) hipLaunchKernelGGL(k_fix_clear, dim3(nreps), dim3(kBlock), 0, s, reps, rep_list);
+}
+void launch_fix_initial(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_fix_initial, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, fmask);
+}
+void launch_fix_two_wall(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_fix_two_wall, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, fmask);
+}
+void launch_fix_reduce(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, unsigned cmask,
+                       int phase, hipStream_t s)
+{
+    if (nitems)
+        hipLaunchKernelGGL(k_fix_reduce, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, fmask, cmask, phase);
+}
+void launch_fix_post_force(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_fix_post_force, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, fmask);
+}
+void launch_fix_final(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_fix_final, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, fmask);
+}
+void launch_fix_end(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s)
+{
+    if (nitems) hipLaunchKernelGGL(k_fix_end, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, fmask);
+}
+void launch_fix_records(const RepDev *reps, const int *rep_list, int nreps, unsigned fmask, unsigned cmask,
+                        long long step, hipStream_t s)
+{
+    if (nreps) hipLaunchKernelGGL(k_fix_records, dim3(nreps), dim3(64), 0, s, reps, rep_list, fmask, cmask, step);
 }
 void launch_nve(const RepDev *reps, const WorkItem *items, int nitems, int initial, hipStream_t s)
 {

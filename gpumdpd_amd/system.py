@@ -25,6 +25,14 @@ def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
 
 
+def type_mask(types):
+    """Group of bead types as the ABI's bitmask (bit t-1 for type t)."""
+    m = 0
+    for t in types:
+        m |= 1 << (int(t) - 1)
+    return m
+
+
 class Context:
     """One device + one HIP stream (libgdpd gdpd_ctx)."""
 
@@ -212,6 +220,69 @@ class System:
         n = C.c_int()
         self.ctx.check(self.lib.gdpd_thermo_rows(self.ctx.h, self.sid, _ptr(out, C.c_double), max_rows,
                                                  C.byref(n)))
+        return out[: n.value].copy()
+
+    # ---- group fixes and computes ----
+    def set_fixes(self, fixes):
+        """fixes: dicts as in oracle-style specs: kind, group (types), and
+        move v; wall_reflect lo, hi, dir; two_wall lo_group, hi_group, dir;
+        spring k, xyz, r0, every; recenter every, xyz; rvcm every, flags."""
+        arr = (L.Fix * max(1, len(fixes)))()
+        for k, f in enumerate(fixes):
+            a = arr[k]
+            a.kind = L.FIX_KINDS[f["kind"]]
+            a.group = type_mask(f["group"])
+            a.dir = int(f.get("dir", 0))
+            a.every = int(f.get("every", 0))
+            kind = f["kind"]
+            if kind == "move":
+                a.p[0], a.p[1], a.p[2] = f["v"]
+            elif kind == "wall_reflect":
+                a.p[0], a.p[1] = f["lo"], f["hi"]
+            elif kind == "two_wall":
+                a.group2, a.group3 = type_mask(f["lo_group"]), type_mask(f["hi_group"])
+            elif kind == "spring":
+                a.p[0] = f["k"]
+                a.p[1], a.p[2], a.p[3] = f["xyz"]
+                a.p[4] = f["r0"]
+            elif kind == "recenter":
+                a.p[0], a.p[1], a.p[2] = f["xyz"]
+            elif kind == "rvcm":
+                a.flags = sum(1 << d for d in range(3) if f["flags"][d])
+        self.ctx.check(self.lib.gdpd_set_fixes(self.ctx.h, self.sid, arr, len(fixes)))
+
+    def set_computes(self, computes):
+        arr = (L.Compute * max(1, len(computes)))()
+        for k, c in enumerate(computes):
+            a = arr[k]
+            a.kind = L.COMPUTE_KINDS[c["kind"]]
+            a.group = type_mask(c["group"])
+            a.every = int(c["every"])
+            if c["kind"] == "rho":
+                for q in range(6):
+                    a.p[q] = c["region"][q]
+            elif c["kind"] == "wall_contact":
+                a.group2 = type_mask(c["wall_group"])
+                a.side, a.dir, a.p[0] = int(c["side"]), int(c["dir"]), c["cutoff"]
+        self.ctx.check(self.lib.gdpd_set_computes(self.ctx.h, self.sid, arr, len(computes)))
+
+    def fix_hook(self, index, hook, step):
+        out = np.zeros(5)
+        self.ctx.check(self.lib.gdpd_fix_hook(self.ctx.h, self.sid, int(index), L.HOOKS[hook], int(step),
+                                              _ptr(out, C.c_double)))
+        return out
+
+    def compute_eval(self, index):
+        out = np.zeros(3)
+        self.ctx.check(self.lib.gdpd_compute_eval(self.ctx.h, self.sid, int(index), _ptr(out, C.c_double)))
+        return out
+
+    def records(self, index, fix=False, max_rows=L.RECORD_CAP):
+        out = np.empty((max_rows, L.RECORD_LEN))
+        n = C.c_int()
+        slot = (L.MAX_COMPUTES + index) if fix else index
+        self.ctx.check(self.lib.gdpd_records(self.ctx.h, self.sid, slot, _ptr(out, C.c_double), max_rows,
+                                             C.byref(n)))
         return out[: n.value].copy()
 
     def step(self):

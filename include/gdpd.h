@@ -156,6 +156,78 @@ int gdpd_thermo_rows(gdpd_ctx *ctx, int sys, double *out, int max_rows, int *n_r
 /* step counter of the next force evaluation */
 int gdpd_get_step(gdpd_ctx *ctx, int sys, int64_t *step);
 
+/* ---- group fixes and computes ------------------------------------------------- */
+/* A group is a set of bead types (system/system.cu:235-301 selects groups by type
+   only), passed as a bitmask: bit (t - 1) set for type t.  Fixes act in the order
+   given at the reference's hook points (main/utils.cuh:251-313): initial_integrate
+   (nve, move), post_integrate (wall_reflect, two_wall), post_force (spring),
+   final_integrate (nve) and end_of_step (rvcm, recenter).  Without any
+   GDPD_FIX_NVE entry every bead is integrated by nve, as before. */
+#define GDPD_FIX_NVE 1          /* fix ID grp nve                          fix/fix_nve.cu:93-177 */
+#define GDPD_FIX_MOVE 2         /* fix ID grp move vx vy vz                fix/fix_move.cu:45-127 */
+#define GDPD_FIX_WALL_REFLECT 3 /* fix ID grp wall_reflect lo hi dir       fix/fix_wall_reflect.cu:79-146 */
+#define GDPD_FIX_TWO_WALL 4     /* fix ID grp two_wall lo_grp hi_grp dir   fix/fix_two_wall.cu:79-204 */
+#define GDPD_FIX_SPRING 5       /* fix ID grp spring k x y z r0            fix/fix_spring.cu:210-376 */
+#define GDPD_FIX_RECENTER 6     /* fix ID grp recenter every x y z         fix/fix_recenter.cu:160-340 */
+#define GDPD_FIX_RVCM 7         /* fix ID grp rvcm every fx fy fz          fix/fix_rvcm.cu:160-292 */
+#define GDPD_MAX_FIXES 8
+
+typedef struct {
+    int kind;         /* GDPD_FIX_* */
+    uint32_t group;   /* the fix's group */
+    uint32_t group2;  /* two_wall: the lower wall's group */
+    uint32_t group3;  /* two_wall: the upper wall's group */
+    int every;        /* recenter, rvcm: act when step % every == 0 */
+    int dir;          /* wall_reflect, two_wall: 0 x, 1 y, 2 z */
+    int flags;        /* rvcm: bit 0 x, bit 1 y, bit 2 z */
+    double p[5];      /* move {vx,vy,vz}; wall_reflect {lo,hi}; spring {k,x,y,z,r0};
+                         recenter {x,y,z} */
+} gdpd_fix;
+
+/* replaces preprocessFixes / enableFix (fix/fix.cu:40-120) for the hot-path fixes */
+int gdpd_set_fixes(gdpd_ctx *ctx, int sys, const gdpd_fix *fixes, int n_fixes);
+
+#define GDPD_HOOK_INITIAL_INTEGRATE 0
+#define GDPD_HOOK_POST_INTEGRATE 1
+#define GDPD_HOOK_POST_FORCE 2
+#define GDPD_HOOK_FINAL_INTEGRATE 3
+#define GDPD_HOOK_END_OF_STEP 4
+/* One fix's hook method on the current state (Fix::initial_integrate ... end_of_step,
+   fix/fix.cuh:20-25).  step selects the every-N fixes.  out (may be NULL) gets the
+   spring's {fx, fy, fz, signed |f|, energy} of that call (fix_spring.cu:330-350). */
+int gdpd_fix_hook(gdpd_ctx *ctx, int sys, int index, int hook, uint32_t step, double *out5);
+
+#define GDPD_COMPUTE_TEMP 1         /* compute ID grp temp every file             compute/compute_temp.cu */
+#define GDPD_COMPUTE_MSD 2          /* compute ID grp msd every file              compute/compute_msd.cu */
+#define GDPD_COMPUTE_RHO 3          /* compute ID grp rho every xlo xhi ylo yhi zlo zhi file */
+#define GDPD_COMPUTE_TF 4           /* compute ID grp tf every file               compute/compute_tf.cu */
+#define GDPD_COMPUTE_WALL_CONTACT 5 /* compute ID grp wall_contact every wgrp lo|hi dir cutoff file */
+#define GDPD_MAX_COMPUTES 8
+
+typedef struct {
+    int kind;         /* GDPD_COMPUTE_* */
+    uint32_t group;
+    uint32_t group2;  /* wall_contact: the wall's group */
+    int every;
+    int dir;          /* wall_contact axis */
+    int side;         /* wall_contact: +1 lo wall, -1 hi wall */
+    double p[6];      /* rho {xlo,xhi,ylo,yhi,zlo,zhi}; wall_contact {cutoff} */
+} gdpd_compute;
+
+/* replaces preprocessCompute / enableCompute (compute/compute.cu:30-110).  msd takes
+   its reference positions (unwrapped) from the state at this call. */
+int gdpd_set_computes(gdpd_ctx *ctx, int sys, const gdpd_compute *computes, int n_computes);
+/* Compute::compute on the current state: out gets up to 3 values (temp; msd; rho;
+   tf x y z; wall_contact count), like the reference's file row for that step. */
+int gdpd_compute_eval(gdpd_ctx *ctx, int sys, int index, double *out3);
+/* Records of the fused run (gdpd_run / zero_step / advance / finish), oldest first:
+   GDPD_RECORD_LEN doubles each, {step, values...}.  index < GDPD_MAX_COMPUTES names
+   a compute; GDPD_MAX_COMPUTES + k names fix k (spring: {step, fx, fy, fz, f, e},
+   one record per step).  A ring of GDPD_RECORD_CAP records per index. */
+#define GDPD_RECORD_LEN 6
+#define GDPD_RECORD_CAP 1024
+int gdpd_records(gdpd_ctx *ctx, int sys, int index, double *out, int max_rows, int *n_rows);
+
 /* ---- batched replicas (no reference counterpart: the reference runs one system) */
 /* Advances every listed system by nsteps whole steps, all systems' bricks in the
    same launches (one launch per phase per step, graph-captured). */

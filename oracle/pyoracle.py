@@ -234,3 +234,209 @@ def run(cfg, nsteps, log_every=100, check_every=1):
         raise RuntimeError("oracle run: neighbour list overflow")
     k = s._keep
     return rows[:nr], {"pos": k["pos"], "vel": k["vel"], "uwpos": k["uwpos"], "force": force}, builds.value
+
+
+# ---------------------------------------------------------------------------
+# Group fixes and computes: a restatement of fix/fix_*.cu and compute/compute_*.cu
+# in the reference's step order (main/utils.cuh:166-314), on id-ordered arrays.
+# Groups are sets of types (system/system.cu:235-301).  Hook order inside one
+# hook point: the order of `fixes`, with post_integrate's wall_reflect fixes
+# before its two_wall fixes (the reference walks an unordered_map there).
+# ---------------------------------------------------------------------------
+
+def _wrap(x, lo, hi):
+    """fix/fix_nve.cu:68-76 (and fix_move.cu, fix_recenter.cu): shift by whole
+    box lengths until lo <= x < hi, one coordinate array at a time."""
+    x = np.array(x, dtype=np.float64)
+    L = hi - lo
+    while True:
+        up, dn = x >= hi, x < lo
+        if not (up.any() or dn.any()):
+            return x
+        x = x - L * (up.astype(np.float64) - dn.astype(np.float64))
+
+
+def _in_group(types, group):
+    return np.isin(types, sorted(group))
+
+
+def _reflect(x, v, lo, hi):
+    """fix_wall_reflect.cu:96-118 / fix_two_wall.cu:118-140 on one axis."""
+    x, v = x.copy(), v.copy()
+    below, above = x < lo, (x > hi) & ~(x < lo)
+    disp_b, disp_a = 2.0 * (lo - x[below]), 2.0 * (hi - x[above])
+    x[below] += disp_b
+    v[below] = -v[below]
+    x[above] += disp_a
+    v[above] = -v[above]
+    d = np.zeros_like(x)
+    d[below], d[above] = disp_b, disp_a
+    return x, v, d
+
+
+def _spring_terms(fx, com):
+    """fix_spring.cu:300-350: spring on the group's centre of mass."""
+    k, c, r0 = fx["k"], np.asarray(fx["xyz"], dtype=np.float64), fx["r0"]
+    d = com - c
+    r = float(np.sqrt(d @ d))
+    dr = r - r0
+    if r > 1.0e-10:
+        f = -k * dr * d / r
+        fm = float(np.sqrt(f @ f))
+        return f, (-fm if dr < 0.0 else fm), 0.5 * k * dr * dr
+    return np.zeros(3), 0.0, 0.0
+
+
+def run_fixes(cfg, nsteps, fixes, computes=(), log_every=1):
+    """Reference-order run with group fixes and computes, FP64.
+
+    fixes: dicts with "kind" in nve, move, wall_reflect, two_wall, spring,
+    recenter, rvcm; "group" (set of types) and the kind's parameters
+    (move: v; wall_reflect: lo, hi, dir; two_wall: lo_group, hi_group, dir;
+    spring: k, xyz, r0, every (records; 0 none); recenter: every, xyz;
+    rvcm: every, flags (x, y, z)).
+    computes: dicts with "kind" in temp, msd, rho, tf, wall_contact, "group",
+    "every" and (rho) region (xlo, xhi, ylo, yhi, zlo, zhi) or (wall_contact)
+    wall_group, side (+1 lo, -1 hi), dir, cutoff.
+
+    Returns (thermo rows (n, 13), records {("compute", k) | ("fix", k): list of
+    rows [step, v...]}, final state dict by id-1).
+    """
+    N = cfg["N"]
+    ids = np.asarray(cfg["id"], dtype=np.int64)
+    order = np.argsort(ids)
+    types = np.asarray(cfg["type"], dtype=np.int64)[order]
+    pos = np.asarray(cfg["pos"], dtype=np.float64)[order].copy()
+    vel = np.asarray(cfg["vel"], dtype=np.float64)[order].copy()
+    lo, hi = np.asarray(cfg["lo"], dtype=np.float64), np.asarray(cfg["hi"], dtype=np.float64)
+    L = hi - lo
+    vol = float(np.prod(L))
+    masses = np.asarray(cfg["masses"], dtype=np.float64)
+    m = masses[types - 1]
+    dt = cfg["dt"]
+    pos = np.stack([_wrap(pos[:, d], lo[d], hi[d]) for d in range(3)], axis=1)
+    uw = pos.copy()
+    x0 = uw.copy()
+    sorted_cfg = dict(cfg, id=np.arange(1, N + 1, dtype=np.int32), type=types.astype(np.int32))
+    rec = {}
+    rows = []
+    gmask = [_in_group(types, f["group"]) for f in fixes]
+    cmask = [_in_group(types, c["group"]) for c in computes]
+
+    def force(step):
+        f, pe, bpe, vir, _ = forces(sorted_cfg, pos=pos, vel=vel, step=step)
+        return f, pe, bpe, vir
+
+    def com(g):
+        return (m[g, None] * uw[g]).sum(axis=0) / m[g].sum()
+
+    def post_force(step, f):
+        for k, fx in enumerate(fixes):
+            if fx["kind"] != "spring":
+                continue
+            g = gmask[k]
+            F, fs, e = _spring_terms(fx, com(g))
+            f[g] += (F / m[g].sum())[None, :] * m[g, None]
+            if fx.get("every", 0) > 0 and step % fx["every"] == 0:
+                rec.setdefault(("fix", k), []).append([step, F[0], F[1], F[2], fs, e])
+
+    def thermo(step, pe, bpe, vir):
+        ke = 0.5 * float((m[:, None] * vel * vel).sum())
+        mvv = [(m * vel[:, a] * vel[:, b]).sum() for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+        p6 = [(mvv[c] + vir[:, c].sum()) / vol for c in range(6)]
+        rows.append([2.0 * ke / (3.0 * N - 3.0), pe.sum(), bpe, (p6[0] + p6[3] + p6[5]) / 3.0] + p6
+                    + [ke, float(step), 0.0])
+
+    def do_computes(step, f):
+        for k, c in enumerate(computes):
+            if step % c["every"] != 0:
+                continue
+            g = cmask[k]
+            kind = c["kind"]
+            if kind == "temp":
+                ke = 0.5 * float((m[g, None] * vel[g] ** 2).sum())
+                val = [2.0 * ke / (3.0 * g.sum() - 3.0)]
+            elif kind == "msd":
+                val = [float(((uw[g] - x0[g]) ** 2).sum()) / g.sum()]
+            elif kind == "rho":
+                r = c["region"]
+                inside = g.copy()
+                for d in range(3):
+                    inside &= (pos[:, d] >= r[2 * d]) & (pos[:, d] < r[2 * d + 1])
+                val = [inside.sum() / ((r[1] - r[0]) * (r[3] - r[2]) * (r[5] - r[4]))]
+            elif kind == "tf":
+                val = list(f[g].sum(axis=0))
+            elif kind == "wall_contact":
+                w = _in_group(types, c["wall_group"])
+                xd = pos[:, c["dir"]]
+                if c["side"] > 0:
+                    b = xd[w].max()
+                    inside = g & (xd >= b) & (xd < b + c["cutoff"])
+                else:
+                    b = xd[w].min()
+                    inside = g & (xd >= b - c["cutoff"]) & (xd < b)
+                val = [float(inside.sum())]
+            rec.setdefault(("compute", k), []).append([step] + val + [0.0] * (5 - len(val)))
+
+    # zero_step (main/utils.cuh:166-205)
+    f, pe, bpe, vir = force(0)
+    post_force(0, f)
+    if log_every > 0:
+        thermo(0, pe, bpe, vir)
+    do_computes(0, f)
+    for step in range(1, nsteps + 1):
+        # initial_integrate: nve (fix_nve.cu:30-77), move (fix_move.cu:45-90)
+        for k, fx in enumerate(fixes):
+            g = gmask[k]
+            if fx["kind"] == "nve":
+                vel[g] = vel[g] + ((0.5 * dt) / m[g])[:, None] * f[g]
+                for d in range(3):
+                    pos[g, d] = _wrap(pos[g, d] + dt * vel[g, d], lo[d], hi[d])
+                uw[g] += dt * vel[g]
+            elif fx["kind"] == "move":
+                vm = np.asarray(fx["v"], dtype=np.float64)
+                vel[g] = vm
+                for d in range(3):
+                    pos[g, d] = _wrap(pos[g, d] + vm[d] * dt, lo[d], hi[d])
+                uw[g] += vm * dt
+        # post_integrate: wall_reflect, then two_wall
+        for k, fx in enumerate(fixes):
+            if fx["kind"] != "wall_reflect":
+                continue
+            g, d = gmask[k], fx["dir"]
+            x, v, disp = _reflect(pos[g, d], vel[g, d], fx["lo"], fx["hi"])
+            pos[g, d], vel[g, d] = x, v
+            uw[g, d] += disp
+        for k, fx in enumerate(fixes):
+            if fx["kind"] != "two_wall":
+                continue
+            g, d = gmask[k], fx["dir"]
+            wlo = pos[_in_group(types, fx["lo_group"]), d].max()
+            whi = pos[_in_group(types, fx["hi_group"]), d].min()
+            x, v, disp = _reflect(pos[g, d], vel[g, d], wlo, whi)
+            pos[g, d], vel[g, d] = x, v
+            uw[g, d] += disp
+        f, pe, bpe, vir = force(step)
+        post_force(step, f)
+        # final_integrate: nve (fix_nve.cu:120-152)
+        for k, fx in enumerate(fixes):
+            if fx["kind"] == "nve":
+                g = gmask[k]
+                vel[g] = vel[g] + ((0.5 * dt) / m[g])[:, None] * f[g]
+        # end_of_step: rvcm (fix_rvcm.cu:220-292), recenter (fix_recenter.cu:260-340)
+        for k, fx in enumerate(fixes):
+            if fx["kind"] not in ("rvcm", "recenter") or step % fx["every"] != 0:
+                continue
+            g = gmask[k]
+            if fx["kind"] == "rvcm":
+                vcm = (m[g, None] * vel[g]).sum(axis=0) / m[g].sum()
+                vel[g] -= np.asarray(fx["flags"], dtype=np.float64) * vcm
+            else:
+                shift = np.asarray(fx["xyz"], dtype=np.float64) - com(g)
+                for d in range(3):
+                    pos[g, d] = _wrap(pos[g, d] + shift[d], lo[d], hi[d])
+                uw[g] += shift
+        if log_every > 0 and step % log_every == 0:
+            thermo(step, pe, bpe, vir)
+        do_computes(step, f)
+    return np.array(rows), rec, {"pos": pos, "vel": vel, "uwpos": uw, "force": f}
