@@ -136,10 +136,9 @@ void read_input(System &s, const std::string &text)
             s.run_p.dumpforce_flag = a[3] == "force_yes";
             s.run_p.dumpvel_flag = a[4] == "vel_yes";
             s.run_p.write_dump_flag = true;
-        } else if (key == "compute") {
-            throw Error("\n   Error: " + key + " is not supported by this build\n   Line: " + l);
         } else if (key != "pair_style" && key != "pair_coeff" && key != "bond_style" && key != "bond_coeff" &&
-                   key != "angle_style" && key != "angle_coeff" && key != "group" && key != "fix") {
+                   key != "angle_style" && key != "angle_coeff" && key != "group" && key != "fix" &&
+                   key != "compute") {
             // main/read_in.cu:295-305
             throw Error("unknown command " + key);
         }
@@ -801,24 +800,44 @@ void enableFix(const std::string &id, const std::string &groupid, const std::str
     en[id] = std::move(f);
 }
 
+std::vector<GroupFix *> &fixOrder()
+{
+    static std::vector<GroupFix *> v;
+    return v;
+}
+
 void preprocessFixes(System &system)
 {
     getEnabledFixRegistry().clear();
+    fixOrder().clear();
+    std::vector<std::string> ids;
     for_lines(system.input, [&](const std::string &, const std::string &key, const std::vector<std::string> &a) {
         if (key != "fix") return;
         if (a.size() < 3) throw Error("illegal fix command: fix ID groupID style params");
         enableFix(a[0], a[1], a[2], std::vector<std::string>(a.begin() + 3, a.end()), system);
+        ids.push_back(a[0]);
     });
-    for (auto &kv : getEnabledFixRegistry()) kv.second->preprocess(system);
+    // every fix goes to the device in deck order; its index is its position
+    std::vector<gdpd_fix> dev;
+    for (const auto &id : ids) {
+        Fix *f = getEnabledFixRegistry()[id].get();
+        auto *g = dynamic_cast<GroupFix *>(f);
+        if (!g) throw Error("fix " + id + " is not one of this build's fixes");
+        g->index = (int)dev.size();
+        dev.push_back(g->describe(system));
+        fixOrder().push_back(g);
+    }
+    for (GroupFix *g : fixOrder()) g->preprocess(system);
+    if (system.handle >= 0) check(system, gdpd_set_fixes(system.ctx, system.handle, dev.data(), (int)dev.size()));
 }
 void postprocessFixes(System &system)
 {
-    for (auto &kv : getEnabledFixRegistry()) kv.second->postprocess(system);
+    for (GroupFix *g : fixOrder()) g->postprocess(system);
 }
 #define GDPD_FIX_HOOK(name)                                                       \
     void Fix_##name(System &system, unsigned int step)                           \
     {                                                                             \
-        for (auto &kv : getEnabledFixRegistry()) kv.second->name(system, step);   \
+        for (GroupFix *g : fixOrder()) g->name(system, step);                     \
     }
 GDPD_FIX_HOOK(initial_integrate)
 GDPD_FIX_HOOK(post_integrate)
@@ -827,22 +846,556 @@ GDPD_FIX_HOOK(post_force)
 GDPD_FIX_HOOK(end_of_step)
 #undef GDPD_FIX_HOOK
 
+// ---- groups (system/system.cu:235-301) -----------------------------------------------
+
+void create_groups(System &s)
+{
+    s.groups.clear();
+    const char *fmt = "group group_name type type1 type2 ...";
+    for_lines(s.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
+        if (key != "group") return;
+        if (a.size() < 3 || a[1] != "type") illegal(l, "illegal group command", fmt);
+        if (s.groups.count(a[0])) throw Error("Group " + a[0] + " already exists");
+        uint32_t mask = 0;
+        for (size_t k = 2; k < a.size(); ++k) {
+            const long t = to_i(a[k], l, "type id", fmt);
+            if (t < 1 || t > s.n_atomtypes) throw Error("Type in group is not included in the data file");
+            if (mask & (1u << (t - 1))) illegal(l, "Duplicate type found: " + a[k], fmt);
+            mask |= 1u << (t - 1);
+        }
+        s.groups[a[0]] = mask;
+    });
+}
+
+uint32_t find_group(const System &s, const std::string &name)
+{
+    auto it = s.groups.find(name);
+    if (it != s.groups.end()) return it->second;
+    if (name == "all") return s.n_atomtypes >= 32 ? ~0u : ((1u << s.n_atomtypes) - 1u);
+    throw Error("Group " + name + " not found");
+}
+
+namespace {
+
+int axis_of(const std::string &a, const std::string &line, const std::string &error, const std::string &fmt)
+{
+    if (a == "x") return 0;
+    if (a == "y") return 1;
+    if (a == "z") return 2;
+    illegal(line, error, fmt);
+}
+
+gdpd_fix fix_of(int kind, uint32_t group)
+{
+    gdpd_fix f{};
+    f.kind = kind;
+    f.group = group;
+    return f;
+}
+
+// one line "%15u\t\t%15g..." like the reference's std::setw(15) << ... << "\t\t" rows
+void row_line(FILE *file, unsigned step, const double *v, int n)
+{
+    if (!file) return;
+    std::fprintf(file, "%15u", step);
+    for (int k = 0; k < n; ++k) std::fprintf(file, "\t\t%15g", v[k]);
+    std::fprintf(file, "\n");
+    std::fflush(file);
+}
+
+template <typename T> void register_fix(const char *name)
+{
+    registerFixType(name, [](const std::string &id, const std::string &gid,
+                             const std::vector<std::string> &p) -> std::unique_ptr<Fix> {
+        return std::make_unique<T>(id, gid, p);
+    });
+}
+
+} // namespace
+
+void GroupFix::forward(System &s, int hook, unsigned int step, double *out)
+{
+    check(s, gdpd_fix_hook(s.ctx, s.handle, index, hook, step, out));
+}
+
 // ---- fix nve (fix/fix_nve.cu) ------------------------------------------------------
 
-nve::nve(std::string id, std::string gid, const std::vector<std::string> &) : Fix(std::move(id), std::move(gid)) {}
+nve::nve(std::string id, std::string gid, const std::vector<std::string> &) : GroupFix(std::move(id), std::move(gid)) {}
 
 void nve::validateParams(const std::vector<std::string> &params)
 {
     if (!params.empty()) illegal("Invalid fix_nve parameters", "illegal fix_nve command", "fix ID groupID nve");
 }
-void nve::initial_integrate(System &s, unsigned int) { check(s, gdpd_nve_initial(s.ctx, s.handle)); }
-void nve::final_integrate(System &s, unsigned int) { check(s, gdpd_nve_final(s.ctx, s.handle)); }
+gdpd_fix nve::describe(const System &s) const { return fix_of(GDPD_FIX_NVE, find_group(s, group_id)); }
+void nve::initial_integrate(System &s, unsigned int step) { forward(s, GDPD_HOOK_INITIAL_INTEGRATE, step); }
+void nve::final_integrate(System &s, unsigned int step) { forward(s, GDPD_HOOK_FINAL_INTEGRATE, step); }
 
-static bool _gdpd_nve_registered = []() {
-    registerFixType("nve", [](const std::string &id, const std::string &gid,
-                              const std::vector<std::string> &p) -> std::unique_ptr<Fix> {
-        return std::make_unique<nve>(id, gid, p);
+// ---- fix move (fix/fix_move.cu:1-40) ------------------------------------------------
+
+move::move(std::string id, std::string gid, const std::vector<std::string> &) : GroupFix(std::move(id), std::move(gid)) {}
+
+void move::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "fix ID groupID move v_x v_y v_z";
+    if (p.size() != 3) illegal("Invalid fix_move parameters", "illegal fix_move command", fmt);
+    for (int d = 0; d < 3; ++d) v[d] = to_f(p[d], "Invalid fix_move parameters", "v_xyz", fmt);
+}
+gdpd_fix move::describe(const System &s) const
+{
+    gdpd_fix f = fix_of(GDPD_FIX_MOVE, find_group(s, group_id));
+    for (int d = 0; d < 3; ++d) f.p[d] = v[d];
+    return f;
+}
+void move::initial_integrate(System &s, unsigned int step) { forward(s, GDPD_HOOK_INITIAL_INTEGRATE, step); }
+
+// ---- fix wall_reflect (fix/fix_wall_reflect.cu:1-75) -----------------------------------
+
+wall_reflect::wall_reflect(std::string id, std::string gid, const std::vector<std::string> &)
+    : GroupFix(std::move(id), std::move(gid)) {}
+
+void wall_reflect::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "fix ID groupID wall_reflect lo_wall_pos hi_wall_pos wall_direction";
+    const char *err = "illegal fix_wall_reflect command";
+    if (p.size() != 3) illegal("Invalid fix_wall_reflect parameters", err, fmt);
+    lo = to_f(p[0], "Invalid fix_wall_reflect parameters", "lo_wall_pos", fmt);
+    hi = to_f(p[1], "Invalid fix_wall_reflect parameters", "hi_wall_pos", fmt);
+    if (lo >= hi) illegal("lo_wall_pos should lower than hi_wall_pos", err, fmt);
+    dir = axis_of(p[2], "Invalid fix_wall_reflect direction", err, fmt);
+}
+void wall_reflect::preprocess(System &s)
+{
+    const double blo[3] = {s.box.xlo, s.box.ylo, s.box.zlo}, bhi[3] = {s.box.xhi, s.box.yhi, s.box.zhi};
+    if (lo < blo[dir] || hi > bhi[dir]) throw Error("Wall position is out of the box!");
+}
+gdpd_fix wall_reflect::describe(const System &s) const
+{
+    gdpd_fix f = fix_of(GDPD_FIX_WALL_REFLECT, find_group(s, group_id));
+    f.dir = dir;
+    f.p[0] = lo;
+    f.p[1] = hi;
+    return f;
+}
+void wall_reflect::post_integrate(System &s, unsigned int step) { forward(s, GDPD_HOOK_POST_INTEGRATE, step); }
+
+// ---- fix two_wall (fix/fix_two_wall.cu:1-75) -------------------------------------------
+
+two_wall::two_wall(std::string id, std::string gid, const std::vector<std::string> &)
+    : GroupFix(std::move(id), std::move(gid)) {}
+
+void two_wall::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "fix ID groupID two_wall lo_wall_name hi_wall_name direction";
+    const char *err = "illegal fix_two_wall command";
+    if (p.size() != 3) illegal("Invalid fix_two_wall parameters", err, fmt);
+    lo_wall = p[0];
+    hi_wall = p[1];
+    dir = axis_of(p[2], "Invalid fix_two_wall direction", err, fmt);
+}
+gdpd_fix two_wall::describe(const System &s) const
+{
+    gdpd_fix f = fix_of(GDPD_FIX_TWO_WALL, find_group(s, group_id));
+    f.group2 = find_group(s, lo_wall);
+    f.group3 = find_group(s, hi_wall);
+    f.dir = dir;
+    return f;
+}
+void two_wall::post_integrate(System &s, unsigned int step) { forward(s, GDPD_HOOK_POST_INTEGRATE, step); }
+
+// ---- fix spring (fix/fix_spring.cu:1-140) ----------------------------------------------
+
+spring::spring(std::string id, std::string gid, const std::vector<std::string> &) : GroupFix(std::move(id), std::move(gid)) {}
+
+void spring::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "fix ID groupID spring spring_k spring_x spring_y spring_z spring_r0 "
+                      "file_name(optional) file_frequency(optional)";
+    const char *bad = "Invalid fix_spring parameters";
+    if (p.size() != 5 && p.size() != 7) illegal(bad, "illegal fix_spring command", fmt);
+    k = to_f(p[0], bad, "spring_k", fmt);
+    x = to_f(p[1], bad, "spring_x", fmt);
+    y = to_f(p[2], bad, "spring_y", fmt);
+    z = to_f(p[3], bad, "spring_z", fmt);
+    r0 = to_f(p[4], bad, "spring_r0", fmt);
+    if (p.size() == 7) {
+        filename = p[5];
+        const long f = to_i(p[6], bad, "file_frequency", fmt);
+        if (f <= 0) illegal(bad, "illegal fix_spring command", fmt);
+        frequency = (unsigned)f;
+    }
+}
+void spring::preprocess(System &s)
+{
+    const double c[3] = {x, y, z};
+    const double blo[3] = {s.box.xlo, s.box.ylo, s.box.zlo}, bhi[3] = {s.box.xhi, s.box.yhi, s.box.zhi};
+    for (int d = 0; d < 3; ++d)
+        if (c[d] < blo[d] || c[d] > bhi[d])
+            throw Error(std::string("spring position is out of the box in ") + "xyz"[d] + " direction!");
+    if (filename.empty()) return;
+    file = std::fopen(filename.c_str(), "w");
+    if (!file) throw Error("cannot open " + filename);
+    std::fprintf(file, "%15s\t\t%15s\t\t%15s\t\t%15s\t\t%15s\t\t%15s\n", "step", "spring_force_x", "spring_force_y",
+                 "spring_force_z", "spring_force", "spring_erergy");
+}
+void spring::postprocess(System &)
+{
+    if (file) std::fclose(file);
+    file = nullptr;
+}
+gdpd_fix spring::describe(const System &s) const
+{
+    gdpd_fix f = fix_of(GDPD_FIX_SPRING, find_group(s, group_id));
+    f.every = (int)frequency;
+    f.p[0] = k;
+    f.p[1] = x;
+    f.p[2] = y;
+    f.p[3] = z;
+    f.p[4] = r0;
+    return f;
+}
+void spring::write_row(unsigned int step, const double *f5) { row_line(file, step, f5, 5); }
+void spring::post_force(System &s, unsigned int step)
+{
+    double out[5];
+    forward(s, GDPD_HOOK_POST_FORCE, step, out);
+    if (frequency && step % frequency == 0) write_row(step, out);
+}
+
+// ---- fix recenter (fix/fix_recenter.cu:1-40) -------------------------------------------
+
+recenter::recenter(std::string id, std::string gid, const std::vector<std::string> &)
+    : GroupFix(std::move(id), std::move(gid)) {}
+
+void recenter::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "fix ID groupID recenter frequency recenter_x recenter_y recenter_z";
+    const char *bad = "Invalid fix_recenter parameters";
+    if (p.size() != 4) illegal(bad, "illegal fix_recenter command", fmt);
+    const long f = to_i(p[0], bad, "frequency", fmt);
+    if (f <= 0) illegal(bad, "illegal fix_recenter command", fmt);
+    frequency = (unsigned)f;
+    x = to_f(p[1], bad, "recenter_x", fmt);
+    y = to_f(p[2], bad, "recenter_y", fmt);
+    z = to_f(p[3], bad, "recenter_z", fmt);
+}
+gdpd_fix recenter::describe(const System &s) const
+{
+    gdpd_fix f = fix_of(GDPD_FIX_RECENTER, find_group(s, group_id));
+    f.every = (int)frequency;
+    f.p[0] = x;
+    f.p[1] = y;
+    f.p[2] = z;
+    return f;
+}
+void recenter::end_of_step(System &s, unsigned int step) { forward(s, GDPD_HOOK_END_OF_STEP, step); }
+
+// ---- fix rvcm (fix/fix_rvcm.cu:1-40) ---------------------------------------------------
+
+rvcm::rvcm(std::string id, std::string gid, const std::vector<std::string> &) : GroupFix(std::move(id), std::move(gid)) {}
+
+void rvcm::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "fix ID groupID rvcm frequency x_flag(0/1) y_flag(0/1) z_flag(0/1)";
+    const char *bad = "Invalid fix_rvcm parameters";
+    if (p.size() != 4) illegal(bad, "illegal fix_rvcm command", fmt);
+    const long f = to_i(p[0], bad, "frequency", fmt);
+    if (f <= 0) illegal(bad, "illegal fix_rvcm command", fmt);
+    frequency = (unsigned)f;
+    for (int d = 0; d < 3; ++d) flags[d] = (int)to_i(p[1 + d], bad, "flag", fmt);
+}
+gdpd_fix rvcm::describe(const System &s) const
+{
+    gdpd_fix f = fix_of(GDPD_FIX_RVCM, find_group(s, group_id));
+    f.every = (int)frequency;
+    // fix_rvcm.cu:160-175 multiplies by the flag: any nonzero flag removes that component
+    for (int d = 0; d < 3; ++d)
+        if (flags[d]) f.flags |= 1 << d;
+    return f;
+}
+void rvcm::end_of_step(System &s, unsigned int step) { forward(s, GDPD_HOOK_END_OF_STEP, step); }
+
+static bool _gdpd_fixes_registered = []() {
+    register_fix<nve>("nve");
+    register_fix<move>("move");
+    register_fix<wall_reflect>("wall_reflect");
+    register_fix<two_wall>("two_wall");
+    register_fix<spring>("spring");
+    register_fix<recenter>("recenter");
+    register_fix<rvcm>("rvcm");
+    return true;
+}();
+
+// ---- computes (compute/compute.cu and compute_*.cu) --------------------------------------
+
+std::unordered_map<std::string, ComputeConstructor> &getComputeRegistry()
+{
+    static std::unordered_map<std::string, ComputeConstructor> r;
+    return r;
+}
+std::unordered_map<std::string, std::unique_ptr<Compute>> &getEnabledComputeRegistry()
+{
+    static std::unordered_map<std::string, std::unique_ptr<Compute>> r;
+    return r;
+}
+std::vector<Compute *> &computeOrder()
+{
+    static std::vector<Compute *> v;
+    return v;
+}
+void registerComputeType(const std::string &typeName, ComputeConstructor constructor)
+{
+    getComputeRegistry()[typeName] = constructor;
+}
+
+void enableCompute(const std::string &id, const std::string &groupid, const std::string &type,
+                   const std::vector<std::string> &params, System &system)
+{
+    auto &en = getEnabledComputeRegistry();
+    if (en.count(id)) throw Error("Error: Compute ID '" + id + "' is already registered!");
+    auto it = getComputeRegistry().find(type);
+    if (it == getComputeRegistry().end()) throw Error("Error: Compute type '" + type + "' not found!");
+    (void)find_group(system, groupid);
+    auto c = it->second(id, groupid, params);
+    c->validateParams(params);
+    en[id] = std::move(c);
+}
+
+void preprocessCompute(System &system)
+{
+    getEnabledComputeRegistry().clear();
+    computeOrder().clear();
+    std::vector<std::string> ids;
+    for_lines(system.input, [&](const std::string &, const std::string &key, const std::vector<std::string> &a) {
+        if (key != "compute") return;
+        if (a.size() < 3) throw Error("illegal compute command: compute ID groupID style params");
+        enableCompute(a[0], a[1], a[2], std::vector<std::string>(a.begin() + 3, a.end()), system);
+        ids.push_back(a[0]);
     });
+    std::vector<gdpd_compute> dev;
+    for (const auto &id : ids) {
+        Compute *c = getEnabledComputeRegistry()[id].get();
+        if (c->on_device()) {
+            c->index = (int)dev.size();
+            dev.push_back(c->describe(system));
+        }
+        computeOrder().push_back(c);
+    }
+    for (Compute *c : computeOrder()) c->preprocess(system);
+    if (system.handle >= 0 && !dev.empty())
+        check(system, gdpd_set_computes(system.ctx, system.handle, dev.data(), (int)dev.size()));
+}
+
+void postprocessCompute(System &system)
+{
+    for (Compute *c : computeOrder()) c->postprocess(system);
+}
+
+void Compute_compute(System &system, unsigned int step)
+{
+    for (Compute *c : computeOrder()) c->compute(system, step);
+}
+
+void Compute::open(const std::string &header)
+{
+    file = std::fopen(filename.c_str(), "w");
+    if (!file) throw Error("cannot open " + filename);
+    std::fprintf(file, "%15s\t\t%s\n", "step", header.c_str());
+}
+void Compute::postprocess(System &)
+{
+    if (file) std::fclose(file);
+    file = nullptr;
+}
+gdpd_compute Compute::describe(const System &s) const
+{
+    gdpd_compute c{};
+    c.kind = device_kind;
+    c.group = find_group(s, group_id);
+    c.every = (int)frequency;
+    return c;
+}
+void Compute::compute(System &s, unsigned int step)
+{
+    if (step % frequency != 0) return;
+    double v[3];
+    check(s, gdpd_compute_eval(s.ctx, s.handle, index, v));
+    write_row(step, v);
+}
+
+namespace {
+
+// frequency and file name of "compute ID group style frequency ... file"
+void freq_file(Compute &c, const std::vector<std::string> &p, size_t n, const char *name, const char *fmt)
+{
+    const std::string bad = std::string("Invalid compute_") + name + " parameters";
+    const std::string err = std::string("illegal compute_") + name + " command";
+    if (p.size() != n) illegal(bad, err, fmt);
+    const long f = to_i(p[0], bad, "frequency", fmt);
+    if (f <= 0) illegal(bad, err, fmt);
+    c.frequency = (unsigned)f;
+    c.filename = p[n - 1];
+}
+
+template <typename T> void register_compute(const char *name)
+{
+    registerComputeType(name, [](const std::string &id, const std::string &gid,
+                                 const std::vector<std::string> &p) -> std::unique_ptr<Compute> {
+        return std::make_unique<T>(id, gid, p);
+    });
+}
+
+std::string col(const char *name)
+{
+    char b[32];
+    std::snprintf(b, sizeof b, "%15s", name);
+    return b;
+}
+
+} // namespace
+
+temp::temp(std::string id, std::string gid, const std::vector<std::string> &) : Compute(std::move(id), std::move(gid))
+{
+    device_kind = GDPD_COMPUTE_TEMP;
+}
+void temp::validateParams(const std::vector<std::string> &p)
+{
+    freq_file(*this, p, 2, "temp", "compute ID   groupID temp frequency filename");
+}
+void temp::preprocess(System &) { open(col("temp")); }
+void temp::write_row(unsigned int step, const double *v) { row_line(file, step, v, 1); }
+
+msd::msd(std::string id, std::string gid, const std::vector<std::string> &) : Compute(std::move(id), std::move(gid))
+{
+    device_kind = GDPD_COMPUTE_MSD;
+}
+void msd::validateParams(const std::vector<std::string> &p)
+{
+    freq_file(*this, p, 2, "msd", "compute ID   groupID msd frequency filename");
+}
+void msd::preprocess(System &) { open(col("msd")); }
+void msd::write_row(unsigned int step, const double *v) { row_line(file, step, v, 1); }
+
+rho::rho(std::string id, std::string gid, const std::vector<std::string> &) : Compute(std::move(id), std::move(gid))
+{
+    device_kind = GDPD_COMPUTE_RHO;
+}
+void rho::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "compute ID   groupID rho frequency xlo xhi ylo yhi zlo zhi filename";
+    freq_file(*this, p, 8, "rho", fmt);
+    for (int q = 0; q < 6; ++q) region[q] = to_f(p[1 + q], "Invalid compute_rho parameters", "region", fmt);
+}
+void rho::preprocess(System &s)
+{
+    const char *n[6] = {"xlo < box.xlo", "xhi > box.xhi", "ylo < box.ylo", "yhi > box.yhi", "zlo < box.zlo",
+                        "zhi > box.zhi"};
+    const double b[6] = {s.box.xlo, s.box.xhi, s.box.ylo, s.box.yhi, s.box.zlo, s.box.zhi};
+    for (int q = 0; q < 6; ++q)
+        if ((q % 2 == 0 && region[q] < b[q]) || (q % 2 == 1 && region[q] > b[q]))
+            illegal(n[q], "illegal compute_rho command", "compute ID   groupID rho frequency xlo xhi ylo yhi zlo zhi filename");
+    open(col("rho"));
+}
+gdpd_compute rho::describe(const System &s) const
+{
+    gdpd_compute c = Compute::describe(s);
+    for (int q = 0; q < 6; ++q) c.p[q] = region[q];
+    return c;
+}
+void rho::write_row(unsigned int step, const double *v) { row_line(file, step, v, 1); }
+
+tf::tf(std::string id, std::string gid, const std::vector<std::string> &) : Compute(std::move(id), std::move(gid))
+{
+    device_kind = GDPD_COMPUTE_TF;
+}
+void tf::validateParams(const std::vector<std::string> &p)
+{
+    freq_file(*this, p, 2, "tf", "compute ID   groupID tf frequency filename");
+}
+void tf::preprocess(System &) { open(col("tf_x") + "\t\t" + col("tf_y") + "\t\t" + col("tf_z")); }
+void tf::write_row(unsigned int step, const double *v) { row_line(file, step, v, 3); }
+
+wall_contact::wall_contact(std::string id, std::string gid, const std::vector<std::string> &) : Compute(std::move(id), std::move(gid))
+{
+    device_kind = GDPD_COMPUTE_WALL_CONTACT;
+}
+void wall_contact::validateParams(const std::vector<std::string> &p)
+{
+    const char *fmt = "compute ID   groupID wall_contact frequency wall_groupname lo/hi wall_direction cutoff filename";
+    const char *bad = "Invalid compute_wall_contact parameters";
+    freq_file(*this, p, 6, "wall_contact", fmt);
+    wall_group = p[1];
+    if (p[2] == "lo") side = 1;
+    else if (p[2] == "hi") side = -1;
+    else illegal(bad, "illegal compute_wall_contact command", fmt);
+    dir = axis_of(p[3], bad, "illegal compute_wall_contact command", fmt);
+    cutoff = to_f(p[4], bad, "cutoff", fmt);
+}
+void wall_contact::preprocess(System &) { open(col("wall_contact")); }
+gdpd_compute wall_contact::describe(const System &s) const
+{
+    gdpd_compute c = Compute::describe(s);
+    c.group2 = find_group(s, wall_group);
+    c.side = side;
+    c.dir = dir;
+    c.p[0] = cutoff;
+    return c;
+}
+void wall_contact::write_row(unsigned int step, const double *v)
+{
+    if (!file) return;
+    std::fprintf(file, "%15u\t\t%15d\n", step, (int)std::lround(v[0]));
+    std::fflush(file);
+}
+
+dump::dump(std::string id, std::string gid, const std::vector<std::string> &) : Compute(std::move(id), std::move(gid)) {}
+void dump::validateParams(const std::vector<std::string> &p)
+{
+    freq_file(*this, p, 2, "dump", "compute ID   groupID dump frequency filename");
+}
+void dump::preprocess(System &)
+{
+    file = std::fopen(filename.c_str(), "w");
+    if (!file) throw Error("cannot open " + filename);
+}
+void dump::compute(System &s, unsigned int step)
+{
+    if (step % frequency != 0 || !file) return;
+    const uint32_t g = find_group(s, group_id);
+    const size_t N = (size_t)s.n_atoms;
+    std::vector<double> pos(3 * N);
+    check(s, gdpd_download(s.ctx, s.handle, pos.data(), nullptr, nullptr, nullptr));
+    std::vector<int> type_by_id(N, 0), mol_by_id(N, 0);
+    for (size_t i = 0; i < N; ++i) {
+        const int id = s.h_id[i];
+        if (id >= 1 && (size_t)id <= N) type_by_id[(size_t)id - 1] = s.h_type[i];
+    }
+    if (s.atom_style != 0 && s.h_mol_id.size() == N) mol_by_id = s.h_mol_id;
+    size_t n = 0;
+    for (size_t i = 0; i < N; ++i) n += (g >> (type_by_id[i] - 1)) & 1u;
+    // compute/compute_dump.cu:60-130
+    std::fprintf(file, "ITEM: TIMESTEP\n%u\nITEM: NUMBER OF ATOMS\n%zu\nITEM: BOX BOUNDS pp pp pp\n", step, n);
+    std::fprintf(file, "%.9f %.9f\n%.9f %.9f\n%.9f %.9f\n", s.box.xlo, s.box.xhi, s.box.ylo, s.box.yhi, s.box.zlo,
+                 s.box.zhi);
+    std::fprintf(file, s.atom_style == 0 ? "ITEM: ATOMS id type x y z\n" : "ITEM: ATOMS id mol type x y z\n");
+    for (size_t i = 0; i < N; ++i) {
+        if (!((g >> (type_by_id[i] - 1)) & 1u)) continue;
+        if (s.atom_style == 0)
+            std::fprintf(file, "%d %d %.9f %.9f %.9f\n", (int)i + 1, type_by_id[i], pos[3 * i], pos[3 * i + 1],
+                         pos[3 * i + 2]);
+        else
+            std::fprintf(file, "%d %d %d %.9f %.9f %.9f\n", (int)i + 1, mol_by_id[i], type_by_id[i], pos[3 * i],
+                         pos[3 * i + 1], pos[3 * i + 2]);
+    }
+    std::fflush(file);
+}
+
+static bool _gdpd_computes_registered = []() {
+    register_compute<temp>("temp");
+    register_compute<msd>("msd");
+    register_compute<rho>("rho");
+    register_compute<tf>("tf");
+    register_compute<wall_contact>("wall_contact");
+    register_compute<dump>("dump");
     return true;
 }();
 
@@ -917,12 +1470,21 @@ void Thermo::process(System &s, unsigned int step)
 
 void preprocess(System &system, Neigh_list &nl, Thermo &thermo)
 {
+    create_groups(system);
     nl.preprocess(system);
     preprocessPairs(system);
     if (system.atom_style >= 1) preprocessBonds(system);
     if (system.atom_style == 2) preprocessAngles(system);
     preprocessFixes(system);
     thermo.preprocess(system);
+    preprocessCompute(system);
+}
+
+void postprocess(System &system, Thermo &thermo)
+{
+    postprocessCompute(system);
+    postprocessFixes(system);
+    thermo.postprocess(system);
 }
 
 // ---- Dump (system/dump.cu) ------------------------------------------------------------
@@ -976,6 +1538,7 @@ void zero_step(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump)
     if (system.atom_style == 2) Angle_compute(system, 0);
     Fix_post_force(system, 0);
     thermo.process(system, 0);
+    Compute_compute(system, 0);
     // main/utils.cuh:200-204
     if (dump && system.run_p.write_dump_flag && system.run_p.dump_start == 0) dump->dump(system, 0);
 }
@@ -1000,6 +1563,7 @@ void simulation(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump)
         Fix_final_integrate(system, step);
         Fix_end_of_step(system, step);
         thermo.process(system, step);
+        Compute_compute(system, step);
         // main/utils.cuh:296-300
         const Run_p &rp = system.run_p;
         if (dump && rp.write_dump_flag && step >= rp.dump_start && (step - rp.dump_start) % rp.dump_f == 0)
@@ -1016,28 +1580,35 @@ void run_deck(System &system, Thermo &thermo, const std::string &deck_text, cons
         return (name.empty() || name[0] == '/' || dir.empty()) ? name : dir + "/" + name;
     };
     read_data(system, resolve(system.run_p.input_data_name));
-    // fix nve integrates every bead: its group must cover all atom types
-    std::map<std::string, std::vector<int>> groups;
-    for_lines(system.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
-        if (key != "group") return;
-        if (a.size() < 3 || a[1] != "type") illegal(l, "illegal group command", "group name type t1 t2 ...");
-        for (size_t k = 2; k < a.size(); ++k) groups[a[0]].push_back((int)to_i(a[k], l, "type", "group name type t1 ..."));
-    });
-    for_lines(system.input, [&](const std::string &l, const std::string &key, const std::vector<std::string> &a) {
-        if (key != "fix" || a.size() < 2 || a[1] == "all") return;
-        auto g = groups.find(a[1]);
-        if (g == groups.end()) illegal(l, "unknown group " + a[1], "group name type t1 t2 ...");
-        for (int t = 1; t <= system.n_atomtypes; ++t)
-            if (std::find(g->second.begin(), g->second.end(), t) == g->second.end())
-                throw Error("fix " + a[0] + ": groups that leave out bead types are not supported by this build");
-    });
     atoms_to_device(system, device);
     Neigh_list nl;
     if (thermo.file_path.empty()) thermo.file_path = resolve("thermo.txt");
+    // compute and fix spring output files are named relative to dir too
+    const std::string input = system.input;
+    {
+        std::stringstream in(input), out;
+        std::string line;
+        while (std::getline(in, line)) {
+            std::string key;
+            auto a = split(line, key);
+            const size_t n = a.size();
+            // the file name is a compute's last argument and a spring's eighth
+            if (!dir.empty() && ((key == "compute" && n >= 5) || (key == "fix" && n == 10 && a[2] == "spring"))) {
+                const size_t f = key == "compute" ? n - 1 : 8;
+                if (a[f][0] != '/') a[f] = dir + "/" + a[f];
+                line = key;
+                for (const auto &t : a) line += " " + t;
+            }
+            out << line << "\n";
+        }
+        system.input = out.str();
+    }
     preprocess(system, nl, thermo);
-    if (fused && !system.run_p.write_dump_flag) {
+    // dumps need the state at the end of a step: decks with a dump run the hook path
+    bool host_dump = system.run_p.write_dump_flag;
+    for (Compute *c : computeOrder()) host_dump = host_dump || !c->on_device();
+    if (fused && !host_dump) {
         run_fused(system, thermo);
-        for (const auto &r : thermo.rows) thermo.write_row(system, r);
     } else {
         Dump dump;
         if (system.run_p.write_dump_flag) dump.preprocess(system, resolve("all.trj"));
@@ -1045,22 +1616,71 @@ void run_deck(System &system, Thermo &thermo, const std::string &deck_text, cons
         simulation(system, nl, thermo, &dump);
         dump.postprocess(system);
     }
-    thermo.postprocess(system);
+    postprocess(system, thermo);
+    system.input = input;
     if (system.run_p.write_data_flag) write_data(system, resolve(system.run_p.out_data_name));
 }
 
 void run_fused(System &system, Thermo &thermo)
 {
-    check(system, gdpd_run(system.ctx, system.handle, (int)system.run_p.nsteps));
-    std::vector<double> rows((size_t)GDPD_THERMO_LEN * 4096);
-    int n = 0;
-    check(system, gdpd_thermo_rows(system.ctx, system.handle, rows.data(), 4096, &n));
-    thermo.rows.clear();
-    for (int k = 0; k < n; ++k) {
-        const double *o = rows.data() + (size_t)k * GDPD_THERMO_LEN;
-        thermo.rows.push_back({o[GDPD_THERMO_STEP], o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8], o[9],
-                               o[GDPD_THERMO_ANGLE_PE]});
+    // thermo rows and fix/compute records wait in device rings of GDPD_RECORD_CAP
+    // (thermo: 1024) entries: advance in chunks that fill at most 1000 of them,
+    // and keep the rows of steps not seen yet after each chunk
+    const unsigned nsteps = system.run_p.nsteps;
+    unsigned span = system.run_p.log_f;
+    for (Compute *c : computeOrder())
+        if (c->on_device()) span = std::min(span, c->frequency);
+    for (GroupFix *g : fixOrder()) {
+        auto *sp = dynamic_cast<spring *>(g);
+        if (sp && sp->frequency) span = std::min(span, sp->frequency);
     }
+    const unsigned chunk = std::max(1u, span * 1000u);
+    double last_row = -1.0;
+    std::vector<double> last_rec(GDPD_MAX_COMPUTES + GDPD_MAX_FIXES, -1.0);
+    std::vector<double> buf((size_t)std::max(GDPD_RECORD_CAP * GDPD_RECORD_LEN, 1024 * GDPD_THERMO_LEN));
+    auto collect = [&]() {
+        int n = 0;
+        check(system, gdpd_thermo_rows(system.ctx, system.handle, buf.data(), 1024, &n));
+        for (int k = 0; k < n; ++k) {
+            const double *o = buf.data() + (size_t)k * GDPD_THERMO_LEN;
+            if (o[GDPD_THERMO_STEP] <= last_row) continue;
+            last_row = o[GDPD_THERMO_STEP];
+            std::vector<double> r = {o[GDPD_THERMO_STEP], o[0], o[1], o[2], o[3], o[4], o[5], o[6], o[7], o[8],
+                                     o[9], o[GDPD_THERMO_ANGLE_PE]};
+            thermo.rows.push_back(r);
+            thermo.write_row(system, r);
+        }
+        auto records = [&](int slot, auto emit) {
+            check(system, gdpd_records(system.ctx, system.handle, slot, buf.data(), GDPD_RECORD_CAP, &n));
+            for (int k = 0; k < n; ++k) {
+                const double *o = buf.data() + (size_t)k * GDPD_RECORD_LEN;
+                if (o[0] <= last_rec[slot]) continue;
+                last_rec[slot] = o[0];
+                emit((unsigned)o[0], o + 1);
+            }
+        };
+        for (Compute *c : computeOrder())
+            if (c->on_device()) records(c->index, [&](unsigned st, const double *v) { c->write_row(st, v); });
+        for (GroupFix *g : fixOrder())
+            if (auto *sp = dynamic_cast<spring *>(g))
+                if (sp->frequency)
+                    records(GDPD_MAX_COMPUTES + g->index, [&](unsigned st, const double *v) { sp->write_row(st, v); });
+    };
+    thermo.rows.clear();
+    if (nsteps == 0) {
+        check(system, gdpd_run(system.ctx, system.handle, 0));
+        collect();
+        return;
+    }
+    check(system, gdpd_zero_step(system.ctx, system.handle));
+    for (unsigned done = 1; done < nsteps;) {
+        const unsigned k = std::min(chunk, nsteps - done);
+        check(system, gdpd_advance(system.ctx, system.handle, (int)k));
+        collect();
+        done += k;
+    }
+    check(system, gdpd_finish(system.ctx, system.handle));
+    collect();
 }
 
 } // namespace gdpd_host

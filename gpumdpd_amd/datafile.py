@@ -122,10 +122,12 @@ def read_data(path):
     return out
 
 
-def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None, dump=None):
+def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None, dump=None, groups_fixes=None):
     """A reference-style run.in for cfg (pair style, coefficients, bonds,
     neighbor, timestep, thermo, run, read_data, group and fix nve); dump is
-    the argument string of a dump command, e.g. "0 5 wrapped force_yes vel_yes"."""
+    the argument string of a dump command, e.g. "0 5 wrapped force_yes vel_yes".
+    groups_fixes (a list of group / fix / compute lines) replaces the default
+    "group all" + "fix md all nve"."""
     bonded = "bonds" in cfg
     angled = "angles" in cfg
     nt = cfg["ntypes"]
@@ -162,7 +164,9 @@ def write_deck(path, cfg, data_name, nsteps, log_every, write_name=None, dump=No
         lines.append("angle_style harmonic")
         ac = np.asarray(cfg["angle_coeff"], dtype=np.float64).reshape(-1, 2)
         lines += [f"angle_coeff {t + 1} {ac[t, 0]:.17g} {ac[t, 1]:.17g}" for t in range(len(ac))]
-    lines += ["group all type " + " ".join(str(t) for t in range(1, nt + 1)), "fix md all nve"]
+    if groups_fixes is None:
+        groups_fixes = ["group all type " + " ".join(str(t) for t in range(1, nt + 1)), "fix md all nve"]
+    lines += list(groups_fixes)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 

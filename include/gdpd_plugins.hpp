@@ -9,6 +9,8 @@
 // same message, so a driver can print it and exit(1) itself.
 #pragma once
 
+#include <cstdint>
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -63,11 +65,18 @@ struct System {
     bool has_velocities = false;  // the data file had a Velocities section
     int precision = GDPD_PREC_F32;
     int h_update_flag = 0;        // Neigh::h_update_flag
+    // system/system.cu:235-301: groups select bead types; a group is kept as a
+    // type mask (bit t-1 for type t).  "all" names every type unless a deck
+    // defines a group of that name.
+    std::map<std::string, uint32_t> groups;
     gdpd_ctx *ctx = nullptr;
     int handle = -1;              // libgdpd system handle
 };
 
 void check(System &s, int rc);
+// system/system.cu:235-301 (create_groups) from the deck's group commands
+void create_groups(System &s);
+uint32_t find_group(const System &s, const std::string &name);
 // read_in: strips comments (main/read_in.cu:5-29) and parses atom_style,
 // neighbor, timestep, thermo, run (main/read_in.cu:40-290)
 void read_input(System &s, const std::string &text);
@@ -215,14 +224,211 @@ void Fix_final_integrate(System &system, unsigned int step);
 void Fix_post_force(System &system, unsigned int step);
 void Fix_end_of_step(System &system, unsigned int step);
 
-// fix nve (fix/fix_nve.cu); this build integrates every bead ("group all")
-class nve : public Fix {
+// The hot-path fixes (fix/fix_*.cu).  preprocessFixes hands every enabled fix to
+// libgdpd in deck order (gdpd_set_fixes); each hook method then forwards to
+// gdpd_fix_hook with the fix's index.  Fix_* walk the fixes in deck order.
+class GroupFix : public Fix {
+public:
+    using Fix::Fix;
+    int index = -1;          // position in the gdpd_set_fixes array
+    virtual gdpd_fix describe(const System &system) const = 0;
+    void forward(System &system, int hook, unsigned int step, double *out = nullptr);
+};
+
+// fix ID group nve (fix/fix_nve.cu)
+class nve : public GroupFix {
 public:
     nve(std::string id, std::string gid, const std::vector<std::string> &params);
     void validateParams(const std::vector<std::string> &params) override;
     std::string getName() override { return "nve"; }
+    gdpd_fix describe(const System &system) const override;
     void initial_integrate(System &system, unsigned int step) override;
     void final_integrate(System &system, unsigned int step) override;
+};
+
+// fix ID group move v_x v_y v_z (fix/fix_move.cu)
+class move : public GroupFix {
+public:
+    move(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "move"; }
+    gdpd_fix describe(const System &system) const override;
+    void initial_integrate(System &system, unsigned int step) override;
+    double v[3] = {0, 0, 0};
+};
+
+// fix ID group wall_reflect lo_wall_pos hi_wall_pos x|y|z (fix/fix_wall_reflect.cu)
+class wall_reflect : public GroupFix {
+public:
+    wall_reflect(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "wall_reflect"; }
+    gdpd_fix describe(const System &system) const override;
+    void preprocess(System &system) override;
+    void post_integrate(System &system, unsigned int step) override;
+    double lo = 0, hi = 0;
+    int dir = 2;
+};
+
+// fix ID group two_wall lo_wall_group hi_wall_group x|y|z (fix/fix_two_wall.cu)
+class two_wall : public GroupFix {
+public:
+    two_wall(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "two_wall"; }
+    gdpd_fix describe(const System &system) const override;
+    void post_integrate(System &system, unsigned int step) override;
+    std::string lo_wall, hi_wall;
+    int dir = 2;
+};
+
+// fix ID group spring k x y z r0 [file frequency] (fix/fix_spring.cu)
+class spring : public GroupFix {
+public:
+    spring(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "spring"; }
+    gdpd_fix describe(const System &system) const override;
+    void preprocess(System &system) override;
+    void postprocess(System &system) override;
+    void post_force(System &system, unsigned int step) override;
+    void write_row(unsigned int step, const double *f5);
+    double k = 0, x = 0, y = 0, z = 0, r0 = 0;
+    std::string filename;
+    unsigned frequency = 0;
+    FILE *file = nullptr;
+};
+
+// fix ID group recenter frequency x y z (fix/fix_recenter.cu)
+class recenter : public GroupFix {
+public:
+    recenter(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "recenter"; }
+    gdpd_fix describe(const System &system) const override;
+    void end_of_step(System &system, unsigned int step) override;
+    unsigned frequency = 1;
+    double x = 0, y = 0, z = 0;
+};
+
+// fix ID group rvcm frequency x_flag y_flag z_flag (fix/fix_rvcm.cu)
+class rvcm : public GroupFix {
+public:
+    rvcm(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "rvcm"; }
+    gdpd_fix describe(const System &system) const override;
+    void end_of_step(System &system, unsigned int step) override;
+    unsigned frequency = 1;
+    int flags[3] = {0, 0, 0};
+};
+
+// fixes in deck order (the reference walks its unordered_map)
+std::vector<GroupFix *> &fixOrder();
+
+// ---- Compute (compute/compute.cuh) --------------------------------------------
+class Compute {
+public:
+    std::string computeID, group_id;
+    Compute(std::string id, std::string gid) : computeID(std::move(id)), group_id(std::move(gid)) {}
+    virtual ~Compute() = default;
+    virtual void validateParams(const std::vector<std::string> &params) = 0;
+    virtual std::string getName() = 0;
+    virtual void preprocess(System &) {}
+    virtual void postprocess(System &);
+    virtual void compute(System &system, unsigned int step);
+    // device description (compute dump has none: it runs on the host)
+    virtual bool on_device() const { return true; }
+    virtual gdpd_compute describe(const System &system) const;
+    // one file row {v0, v1, v2} of step (reference column layout)
+    virtual void write_row(unsigned int step, const double *v) = 0;
+    void open(const std::string &header);
+    int index = -1;          // position in the gdpd_set_computes array
+    int device_kind = 0;     // GDPD_COMPUTE_* (0: host only)
+    unsigned frequency = 1;
+    std::string filename;
+    FILE *file = nullptr;
+};
+using ComputeConstructor = std::unique_ptr<Compute> (*)(const std::string &, const std::string &,
+                                                         const std::vector<std::string> &);
+std::unordered_map<std::string, ComputeConstructor> &getComputeRegistry();
+std::unordered_map<std::string, std::unique_ptr<Compute>> &getEnabledComputeRegistry();
+std::vector<Compute *> &computeOrder();
+void registerComputeType(const std::string &typeName, ComputeConstructor constructor);
+void enableCompute(const std::string &id, const std::string &groupid, const std::string &type,
+                   const std::vector<std::string> &params, System &system);
+void preprocessCompute(System &system);
+void postprocessCompute(System &system);
+void Compute_compute(System &system, unsigned int step);
+
+// compute ID group temp frequency file (compute/compute_temp.cu)
+class temp : public Compute {
+public:
+    temp(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "temp"; }
+    void preprocess(System &system) override;
+    void write_row(unsigned int step, const double *v) override;
+};
+
+// compute ID group msd frequency file (compute/compute_msd.cu)
+class msd : public Compute {
+public:
+    msd(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "msd"; }
+    void preprocess(System &system) override;
+    void write_row(unsigned int step, const double *v) override;
+};
+
+// compute ID group rho frequency xlo xhi ylo yhi zlo zhi file (compute/compute_rho.cu)
+class rho : public Compute {
+public:
+    rho(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "rho"; }
+    void preprocess(System &system) override;
+    gdpd_compute describe(const System &system) const override;
+    void write_row(unsigned int step, const double *v) override;
+    double region[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// compute ID group tf frequency file (compute/compute_tf.cu)
+class tf : public Compute {
+public:
+    tf(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "tf"; }
+    void preprocess(System &system) override;
+    void write_row(unsigned int step, const double *v) override;
+};
+
+// compute ID group wall_contact frequency wall_group lo|hi x|y|z cutoff file
+// (compute/compute_wall_contact.cu)
+class wall_contact : public Compute {
+public:
+    wall_contact(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "wall_contact"; }
+    void preprocess(System &system) override;
+    gdpd_compute describe(const System &system) const override;
+    void write_row(unsigned int step, const double *v) override;
+    std::string wall_group;
+    int side = 1, dir = 2;
+    double cutoff = 1.0;
+};
+
+// compute ID group dump frequency file (compute/compute_dump.cu): the group's
+// beads as LAMMPS text frames, written on the host from a state download
+class dump : public Compute {
+public:
+    dump(std::string id, std::string gid, const std::vector<std::string> &params);
+    void validateParams(const std::vector<std::string> &params) override;
+    std::string getName() override { return "dump"; }
+    void preprocess(System &system) override;
+    void compute(System &system, unsigned int step) override;
+    bool on_device() const override { return false; }
+    void write_row(unsigned int, const double *) override {}
 };
 
 // ---- Neigh_list (force/neigh_list.cuh:5-25) ---------------------------------
@@ -263,8 +469,10 @@ public:
 
 // The reference's driver (main/utils.cuh:126-314) over the shells: preprocess,
 // zero_step, simulation and postprocess.  fused = true runs `run N` through
-// gdpd_run (one launch per step) instead of one call per hook.
+// gdpd_zero_step / gdpd_advance / gdpd_finish in chunks instead of one call per
+// hook; thermo rows and the fix/compute records are collected after each chunk.
 void preprocess(System &system, Neigh_list &nl, Thermo &thermo);
+void postprocess(System &system, Thermo &thermo);
 void zero_step(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump = nullptr);
 void simulation(System &system, Neigh_list &nl, Thermo &thermo, Dump *dump = nullptr);
 void run_fused(System &system, Thermo &thermo);
