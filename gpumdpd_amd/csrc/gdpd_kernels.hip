@@ -458,6 +458,20 @@ __device__ __forceinline__ BrickGeom brick_geom(const RepDev &R, int b)
     return g;
 }
 
+// n / d for 0 <= n < 2^22 and 1 <= d <= 2^10 from a float reciprocal of d:
+// (n + 0.5) / d lies at least 1/(2d) from an integer, far more than the float
+// error of the product, so truncation gives the exact quotient
+__device__ __forceinline__ int small_div(int n, float dinv) { return (int)(((float)n + 0.5f) * dinv); }
+
+// region cell rc -> (rx, ry, rz) with rc = (rx * RY + ry) * RZ + rz
+__device__ __forceinline__ void split3(int rc, int RY, int RZ, float inv_rz, float inv_ryz, int &rx, int &ry, int &rz)
+{
+    rx = small_div(rc, inv_ryz);
+    const int rem = rc - rx * RY * RZ;
+    ry = small_div(rem, inv_rz);
+    rz = rem - ry * RZ;
+}
+
 // Periodic wrap of a cell coordinate in [-n, 2n): returns the cell and sets
 // code to the image shift + 1 (0: one box below, 1: none, 2: one box above).
 __device__ __forceinline__ int wrap_code(int c, int n, int &code)
@@ -498,8 +512,11 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
 {
     using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x;
+    const float inv_rz = 1.0f / (float)g.RZ, inv_ryz = 1.0f / (float)(g.RY * g.RZ);
+    const float inv_ez = 1.0f / (float)g.ext[2], inv_eyz = 1.0f / (float)(g.ext[1] * g.ext[2]);
     for (int rc = t; rc < g.nreg; rc += kBlock) {
-        const int rz = rc % g.RZ, ry = (rc / g.RZ) % g.RY, rx = rc / (g.RZ * g.RY);
+        int rx, ry, rz;
+        split3(rc, g.RY, g.RZ, inv_rz, inv_ryz, rx, ry, rz);
         int wx, wy, wz;
         const int gx = wrap_code(g.c0[0] - 1 + rx, R.nc[0], wx);
         const int gy = wrap_code(g.c0[1] - 1 + ry, R.nc[1], wy);
@@ -535,7 +552,9 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     // of cells inside a class is immaterial), then a scan over the classes
     const int nown = g.ext[0] * g.ext[1] * g.ext[2];
     for (int oc = t; oc < nown; oc += kBlock) {
-        const int rz = 1 + oc % g.ext[2], ry = 1 + (oc / g.ext[2]) % g.ext[1], rx = 1 + oc / (g.ext[2] * g.ext[1]);
+        int rx, ry, rz;
+        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
+        ++rx, ++ry, ++rz;
         const int rc = (rx * g.RY + ry) * g.RZ + rz;
         L.cellpre[oc] = atomicAdd(&L.sigcnt[face_class(g, rx, ry, rz)], L.c_start[rc + 1] - L.c_start[rc]);
     }
@@ -548,8 +567,9 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     }
     __syncthreads();
     for (int oc = t; oc < nown; oc += kBlock) {
-        const int rz = 1 + oc % g.ext[2], ry = 1 + (oc / g.ext[2]) % g.ext[1], rx = 1 + oc / (g.ext[2] * g.ext[1]);
-        L.cellpre[oc] += L.sigcnt[face_class(g, rx, ry, rz)];
+        int rx, ry, rz;
+        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
+        L.cellpre[oc] += L.sigcnt[face_class(g, rx + 1, ry + 1, rz + 1)];
     }
     __syncthreads();
     const int S = L.c_start[g.nreg];

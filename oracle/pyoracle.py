@@ -344,8 +344,9 @@ def run_fixes(cfg, nsteps, fixes, computes=(), log_every=1):
         ke = 0.5 * float((m[:, None] * vel * vel).sum())
         mvv = [(m * vel[:, a] * vel[:, b]).sum() for a, b in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
         p6 = [(mvv[c] + vir[:, c].sum()) / vol for c in range(6)]
+        ape = float(angle_force(cfg["angles"], cfg["angle_coeff"], pos, lo, hi)[1].sum()) if "angles" in cfg else 0.0
         rows.append([2.0 * ke / (3.0 * N - 3.0), pe.sum(), bpe, (p6[0] + p6[3] + p6[5]) / 3.0] + p6
-                    + [ke, float(step), 0.0])
+                    + [ke, float(step), ape])
 
     def do_computes(step, f):
         for k, c in enumerate(computes):

@@ -186,3 +186,56 @@ def test_fix_validation(ctx, cfg):
     with pytest.raises(g.GdpdError):
         s.set_computes([{"kind": "rho", "group": {1}, "every": 1, "region": (0, 20, 0, 10, 0, 14)}])
     s.destroy()
+
+
+def bonded_small():
+    return configs.config_c2(n_chains=250, box=11.0)
+
+
+def angles_small():
+    return configs.config_angles(n_chains=250, box=11.0)
+
+
+def mdpd_slab():
+    return configs.config_c1()
+
+
+OTHER = {
+    # diblock melt: A = 1, B = 2; bonds enter the spring/recenter centres and tf (third law incl. bonds)
+    "bonded": (bonded_small,
+               [{"kind": "nve", "group": {1, 2}},
+                {"kind": "rvcm", "group": {1, 2}, "every": 3, "flags": (1, 1, 1)},
+                {"kind": "recenter", "group": {2}, "every": 4, "xyz": (5.5, 5.5, 5.5)},
+                {"kind": "spring", "group": {1}, "k": 10.0, "xyz": (5.0, 6.0, 5.0), "r0": 0.5, "every": 1}],
+               [{"kind": "msd", "group": {1}, "every": 5}, {"kind": "temp", "group": {2}, "every": 5},
+                {"kind": "tf", "group": {1, 2}, "every": 5}]),
+    "angles": (angles_small,
+               [{"kind": "nve", "group": {1, 2}}, {"kind": "rvcm", "group": {1, 2}, "every": 2, "flags": (0, 1, 1)}],
+               [{"kind": "tf", "group": {1}, "every": 5}]),
+    # many-body slab kept centred, with the liquid density of its middle
+    "mdpd": (mdpd_slab,
+             [{"kind": "nve", "group": {1}},
+              {"kind": "recenter", "group": {1}, "every": 5, "xyz": (20.0, 5.0, 5.0)},
+              {"kind": "rvcm", "group": {1}, "every": 5, "flags": (1, 1, 1)}],
+             [{"kind": "rho", "group": {1}, "every": 5, "region": (17.0, 23.0, 0.0, 10.0, 0.0, 10.0)},
+              {"kind": "tf", "group": {1}, "every": 5}]),
+}
+
+
+@pytest.mark.parametrize("name", list(OTHER))
+def test_fix_path_bonded_angles_mdpd_fp64(ctx, name):
+    make_cfg, fixes, comps = OTHER[name]
+    cfg = make_cfg()
+    n = 15
+    print(f"case: fixes {name} FP64 fused, N={cfg['N']}", flush=True)
+    rows_o, rec_o, st_o = O.run_fixes(cfg, n, fixes, comps, log_every=LOG)
+    s = make(ctx, cfg, fixes, comps, g.PREC_F64)
+    s.run(n)
+    d = s.download()
+    np.testing.assert_allclose(d["pos"], st_o["pos"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(d["vel"], st_o["vel"], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(s.thermo_rows(), rows_o, rtol=1e-9, atol=1e-9)
+    for key in rec_keys(fixes, comps):
+        np.testing.assert_allclose(gpu_records(s, key), np.array(rec_o[key]), rtol=1e-9, atol=1e-8,
+                                   err_msg=str(key))
+    s.destroy()
