@@ -399,13 +399,15 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     GDPD_TAKE(rh, sizeof(real) * (p.style == 2 ? p.smax : 1))
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
     GDPD_TAKE(cinfo, sizeof(int) * p.rmax)
-    GDPD_TAKE(gstart, sizeof(int) * p.rmax)
-    GDPD_TAKE(cellpre, sizeof(int) * p.rmax)
     GDPD_TAKE(sigcnt, sizeof(int) * (kSigClasses + 1))
     GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
     GDPD_TAKE(oslot, sizeof(int) * p.omax)
     GDPD_TAKE(oinfo, sizeof(int) * p.omax)
-    GDPD_TAKE(facc, sizeof(long long) * 3 * p.omax)
+    // gstart and cellpre live only while the brick is staged, before the
+    // per-bead sums are cleared, so they share the sums' space
+    o.gstart = (unsigned)off;
+    o.cellpre = (unsigned)(off + sizeof(int) * p.rmax);
+    GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 2 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 2 * 4 * (size_t)p.rmax)
     GDPD_TAKE(queue, sizeof(unsigned) * (kBlock / 64) * p.qw)
 #undef GDPD_TAKE
     o.total = (unsigned)off;
@@ -653,6 +655,7 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         }
     }
     for (int k = t; k < R.ntypes * R.ntypes; k += kBlock) L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
+    __syncthreads();  // gstart and cellpre (aliased by facc) are dead from here
     for (int k = t; k < 3 * O; k += kBlock) L.facc[k] = 0;
     __syncthreads();
     return O;
