@@ -46,6 +46,7 @@ struct Graph {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int steps = 0;
+    int topo = -1;  // bond/angle level the captured integrate kernel was chosen for
     void release()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -53,6 +54,7 @@ struct Graph {
         exec = nullptr;
         graph = nullptr;
         steps = 0;
+        topo = -1;
     }
 };
 
@@ -338,6 +340,19 @@ int rebin(gdpd_ctx *ctx, Tables &T)
     return check_launch(ctx);
 }
 
+// 0: no system of T has bonds or angles, 1: bonds only, 2: angles too
+int topo_of(gdpd_ctx *ctx, const Tables &T)
+{
+    int t = 0;
+    for (int id : T.key) {
+        const Sys *s = (id >= 0 && id < (int)ctx->sys.size()) ? ctx->sys[id] : nullptr;
+        if (!s) continue;
+        if (s->h.nangles > 0) t = 2;
+        else if (s->h.nbonds > 0) t = std::max(t, 1);
+    }
+    return t;
+}
+
 int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
 {
     T.bl_dpd.perbead = T.bl_mdpd.perbead = ((mode & kModeForceOnly) && (mode & kModeLogForce)) ? 1 : 0;
@@ -364,7 +379,7 @@ int step_once(gdpd_ctx *ctx, Tables &T, int mode)
 {
     int rc;
     if ((rc = force_launch(ctx, T, kModeLogAuto))) return rc;
-    launch_integrate(ctx->d_reps, T.beads, T.n_beads, mode | kModeLogAuto, ctx->stream);
+    launch_integrate(ctx->d_reps, T.beads, T.n_beads, mode | kModeLogAuto, topo_of(ctx, T), ctx->stream);
     launch_step_end(ctx->d_reps, T.rep_list, T.n_reps, 1, 1, 1, ctx->stream);
     if (mode & kModeDrift) {
         // the order of beads inside a cell is whatever the rank atomics gave:
@@ -568,7 +583,8 @@ int maybe_resort(gdpd_ctx *ctx, Sys *s, Tables &T, int steps)
 
 int graph_block(gdpd_ctx *ctx, Graph &G, Tables &T, int steps)
 {
-    if (G.exec && G.steps == steps) {
+    const int topo = topo_of(ctx, T);
+    if (G.exec && G.steps == steps && G.topo == topo) {
         HIPCHK(hipGraphLaunch(G.exec, ctx->stream));
         return GDPD_OK;
     }
@@ -586,6 +602,7 @@ int graph_block(gdpd_ctx *ctx, Graph &G, Tables &T, int steps)
     G.graph = g;
     HIPCHK(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
     G.steps = steps;
+    G.topo = topo;
     HIPCHK(hipGraphLaunch(G.exec, ctx->stream));
     return GDPD_OK;
 }

@@ -29,8 +29,9 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s);
 void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
                   int precision, int style, int mode, hipStream_t s);
-// bonds + integration + re-binning, one thread per slot
-void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, hipStream_t s);
+// bonds + integration + re-binning, one thread per slot; topo: 0 no replica has
+// bonds or angles, 1 bonds only, 2 angles too (selects the kernel instance)
+void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, int topo, hipStream_t s);
 // per replica (rep_list: nreps replica indices)
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
                      int thermo, hipStream_t s, int bricks = 1);

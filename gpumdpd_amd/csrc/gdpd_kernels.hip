@@ -1078,6 +1078,10 @@ __device__ __forceinline__ void angle_on_bead(const RepDev &R, const double *X, 
 // buffer.
 // ---------------------------------------------------------------------------
 
+// TOPO: 0 no replica of the launch has bonds or angles, 1 bonds only, 2 angles
+// too.  Code a launch cannot need is compiled out: the plain-fluid instance
+// holds far fewer registers, so more waves keep HBM busy.
+template <int TOPO>
 __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__ reps,
                                                       const WorkItem *__restrict__ items, int nitems, int mode)
 {
@@ -1100,7 +1104,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
         double fx = R.force[s], fy = R.force[N + s], fz = R.force[2 * N + s];
         double xg = X[s], yg = X[N + s], zg = X[2 * N + s];
         double bpe = 0.0, bw[6] = {0, 0, 0, 0, 0, 0};
-        if (R.nbonds > 0) {
+        if (TOPO >= 1 && R.nbonds > 0) {
             const int idi = R.id[s];
             const int b1 = R.bond_start[idi];
             for (int bb = R.bond_start[idi - 1]; bb < b1; ++bb) {
@@ -1128,7 +1132,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             }
         }
         double ape = 0.0;
-        if (R.nangles > 0) {
+        if (TOPO >= 2 && R.nangles > 0) {
             double fa[3] = {fx, fy, fz};
             const int idi = R.id[s];
             const int a1 = R.angle_start[idi];
@@ -2143,9 +2147,15 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
             hipLaunchKernelGGL((k_brick<float, 1>), dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p, mode);
     }
 }
-void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, hipStream_t s)
+void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, int topo, hipStream_t s)
 {
-    if (nitems) hipLaunchKernelGGL(k_integrate, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, mode);
+    if (!nitems) return;
+    if (topo == 0)
+        hipLaunchKernelGGL(k_integrate<0>, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, mode);
+    else if (topo == 1)
+        hipLaunchKernelGGL(k_integrate<1>, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, mode);
+    else
+        hipLaunchKernelGGL(k_integrate<2>, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, mode);
 }
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
                      int thermo, hipStream_t s, int bricks)
