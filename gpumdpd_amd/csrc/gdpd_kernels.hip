@@ -11,12 +11,12 @@
 //
 // The hot kernel (k_brick) gives each 256-thread workgroup a brick of cells.  It
 // stages the brick plus a one-cell halo from HBM into LDS (positions relative
-// to the brick corner, periodic images applied while staging), builds each owned
-// bead's in-range list in LDS, evaluates every pair's DPD/mDPD force with the
-// Saru random term, adds the bead's bonds, and integrates the bead in the same
-// launch (final half-kick of step n, thermo on log steps, half-kick and drift
-// of step n+1), writing the new state to the other buffer and binning it.
-// Forces are summed per bead in a fixed order in FP64, so the result does not
+// to the brick corner, periodic images applied while staging), finds each owned
+// bead's in-range partners in LDS and evaluates every pair's DPD/mDPD force
+// with the Saru random term.  k_integrate then adds each bead's bonds and angles
+// and integrates it (final half-kick of step n, thermo on log steps, half-kick
+// and drift of step n+1), writing the new state to the other buffer and its new
+// cell.  Per-bead and per-brick sums are 64-bit fixed point, so results do not
 // depend on scheduling or on which other replicas share the launch.
 #include "gdpd_kernels.h"
 #include "gdpd_saru.h"

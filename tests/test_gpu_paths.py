@@ -145,3 +145,23 @@ def test_angle_hook_path_equals_fused_path(ctx):
     assert np.array_equal(a.download()["force"], b.download()["force"])
     a.destroy()
     b.destroy()
+
+
+def test_bonds_added_after_graph_steps(ctx):
+    """The integrate kernel instance depends on whether bonds exist, so a system
+    that gains bonds after graph-captured steps must re-capture its step graph:
+    graph-replayed and eager runs agree bit for bit, and the bonds act."""
+    cfg = configs.config_c2(n_chains=400, box=13.0)
+    plain = {k: v for k, v in cfg.items() if k != "bonds"}
+    out = []
+    for graph, bonds in ((8, True), (1, True), (8, False)):
+        s = g.system_from_config(ctx, plain, precision=g.PREC_F32, log_every=10 ** 6)
+        s.set_tuning(graph_steps=graph)
+        s.run(20)
+        if bonds:
+            s.set_bonds(cfg["bonds"], cfg["n_bond_types"], cfg["bond_coeff"])
+        s.run(20)
+        out.append(state(s))
+        s.destroy()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert not np.array_equal(out[0][0], out[2][0])
