@@ -191,3 +191,68 @@ def test_angles_match_oracle(ctx, prec, tol):
     row = r.thermo_rows()[0]
     r.destroy()
     assert row[12] == pytest.approx(ape.sum(), rel=max(tol, 1e-12))
+
+
+def _tiny(n, box, seed, style="dpd"):
+    """A sparse or ragged DPD/mDPD system in a box of 3-12 cells per axis.  Boxes
+    under 3 reference bins of rc + skin get skin 0 (the reference then rebuilds
+    its list every step), so the oracle's 27-bin stencil never repeats a bin."""
+    cfg = configs.dpd_fluid(n, box, seed)
+    skin = 0.3 if min(box) >= 3.9 else 0.0
+    if style == "dpd":
+        cfg.update(style="dpd", temp=1.0, rc=1.0, saru_seed=42, dt=0.04, skin=skin, global_cut=1.0,
+                   max_rho=6.0, coeff=configs.dpd_coeff(1, {(1, 1): (25.0, 4.5, 1.0)}), name=f"tiny {n}")
+    else:
+        cfg.update(style="mdpd", temp=1.0, rc=1.0, rd=0.75, saru_seed=42, dt=0.01, skin=skin, global_cut=1.0,
+                   max_rho=8.0, coeff=configs.mdpd_coeff(1, {(1, 1): (-40.0, 25.0, 4.5, 1.0, 0.75)}),
+                   name=f"tiny mdpd {n}")
+    return cfg
+
+
+TINY = [
+    # (n, box, style): one bead; a pair across all three periodic faces; sparse
+    # and ragged boxes whose bricks' halos wrap onto their own cells; a dense
+    # 3x3x3-cell box; a ragged box of several bricks; mDPD in a small box
+    (1, (3.0, 3.0, 3.0), "dpd"),
+    (2, (3.0, 3.0, 3.0), "dpd"),
+    (7, (3.2, 3.5, 3.9), "dpd"),
+    (60, (3.0, 4.7, 3.3), "dpd"),
+    (81, (3.0, 3.0, 3.0), "dpd"),
+    (400, (3.7, 11.3, 5.2), "dpd"),
+    (150, (3.4, 3.9, 4.6), "mdpd"),
+]
+
+
+@pytest.mark.parametrize("n,box,style", TINY)
+def test_tiny_and_ragged_boxes(ctx, n, box, style):
+    cfg = _tiny(n, box, seed=100 + n, style=style)
+    if n == 2:
+        # in range only through the periodic images on every axis (r = 0.6)
+        cfg["pos"] = np.array([[0.2, 0.3, 2.9], [2.8, 0.1, 0.3]])
+    print(f"case: tiny n={n} box={box} style={style}", flush=True)
+    for prec, tol in ((g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)):
+        s = g.system_from_config(ctx, cfg, precision=prec)
+        s.pair_compute(step=4, log=True)
+        f = s.download()["force"]
+        pe, vir, _ = s.download_pe_virial()
+        f_ref, pe_ref, _, vir_ref, _ = pyoracle.forces(cfg, step=4)
+        s.destroy()
+        assert rel(f, f_ref) < tol and rel(pe, pe_ref) < tol and rel(vir, vir_ref) < tol
+        if n == 1:
+            assert np.all(f == 0.0) and np.all(pe == 0.0)
+        if n == 2:
+            assert np.linalg.norm(f_ref[0]) > 1.0 and np.allclose(f_ref[0], -f_ref[1])
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=5)
+    s.run(20)
+    st = s.download()
+    rows = s.thermo_rows()
+    s.destroy()
+    ref_rows, ref_state, _ = pyoracle.run(cfg, 20, log_every=5)
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    d = st["pos"] - ref_state["pos"]
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-9
+    assert np.max(np.abs(st["vel"] - ref_state["vel"])) < 1e-8
+    assert len(rows) == len(ref_rows) == 5
+    # one bead has no degrees of freedom left after the momentum term: T is inf in both
+    assert np.allclose(rows[:, :11], ref_rows, rtol=1e-9, atol=1e-9, equal_nan=True)
