@@ -165,3 +165,27 @@ def test_bonds_added_after_graph_steps(ctx):
         s.destroy()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert not np.array_equal(out[0][0], out[2][0])
+
+
+def test_brick_capacity_overflow_is_reported_and_cleared_by_upload(ctx):
+    """LDS capacities are sized from the state at the first binning.  A later
+    state that packs far more beads into one brick than that makes the brick
+    kernel stop before staging (it checks the counts first) and set a sticky
+    flag, which the call reports as GDPD_ERR_OVERFLOW; a fresh upload clears it."""
+    from gpumdpd_amd import _lib
+    cfg = configs.scaled_dpd(6000, seed=31)
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=10 ** 6)
+    s.run(2)
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    packed = np.asarray(cfg["lo"]) + 0.5 * L + np.random.default_rng(3).random((cfg["N"], 3)) * 1.5
+    s.upload(cfg["id"], cfg["type"], packed, cfg["vel"], cfg["masses"])
+    with pytest.raises(_lib.GdpdError) as e:
+        s.pair_compute(step=0, log=False)
+    assert e.value.code == 3 and "LDS capacity" in str(e.value)
+    s.upload(cfg["id"], cfg["type"], cfg["pos"], cfg["vel"], cfg["masses"])
+    s.run(2)
+    ref = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=10 ** 6)
+    ref.run(2)
+    assert np.array_equal(state(s)[0], state(ref)[0]) and np.array_equal(state(s)[1], state(ref)[1])
+    s.destroy()
+    ref.destroy()
