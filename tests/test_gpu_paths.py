@@ -189,3 +189,29 @@ def test_brick_capacity_overflow_is_reported_and_cleared_by_upload(ctx):
     assert np.array_equal(state(s)[0], state(ref)[0]) and np.array_equal(state(s)[1], state(ref)[1])
     s.destroy()
     ref.destroy()
+
+
+def test_batch_with_bonded_and_angle_members_equals_alone(ctx):
+    """A batch runs the integrate instance of its most complex member (here one
+    with angles) for every member: plain, bonded and angle replicas still give
+    the same bits as alone."""
+    reqs = [configs.replica_system("dpd", 4000, 8), configs.config_c2(n_chains=300, box=12.0),
+            configs.config_angles(n_chains=250, box=11.0)]
+    alone = []
+    for cfg in reqs:
+        s = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=5)
+        s.zero_step()
+        s.advance(20)
+        s.finish()
+        alone.append(state(s) + (s.thermo_rows(),))
+        s.destroy()
+    sys_ = [g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=5) for cfg in reqs]
+    for s in sys_:
+        s.zero_step()
+    ctx.batch_advance(sys_, 20)
+    for s, ref in zip(sys_, alone):
+        s.finish()
+        p, v = state(s)
+        assert np.array_equal(p, ref[0]) and np.array_equal(v, ref[1])
+        assert np.array_equal(s.thermo_rows(), ref[2])
+        s.destroy()
