@@ -482,6 +482,9 @@ int prepare(gdpd_ctx *ctx, Sys *s)
     h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
     if ((h.bdim[0] + 2) * (h.bdim[1] + 2) * (h.bdim[2] + 2) > 4 * kBlock)
         return fail(ctx, GDPD_ERR_ARG, "brick too large: at most %d region cells", 4 * kBlock);
+    // the owned-bead info word holds region cell coordinates of 5, 5 and 6 bits
+    if (h.bdim[0] > 30 || h.bdim[1] > 30 || h.bdim[2] > 62)
+        return fail(ctx, GDPD_ERR_ARG, "brick edges must be at most 30 x 30 x 62 cells");
 
     int rc;
     if ((rc = dalloc(ctx, s, &h.cell_start, (size_t)h.ncell + 1))) return rc;
@@ -888,6 +891,7 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
 {
     if (!ctx || !d || !sys_out) return fail(ctx, GDPD_ERR_ARG, "null argument");
     if (d->n_beads <= 0) return fail(ctx, GDPD_ERR_ARG, "n_beads must be positive");
+    if (d->n_beads > kMaxBeads) return fail(ctx, GDPD_ERR_ARG, "at most %d beads per system", kMaxBeads);
     if (d->n_types < 1 || d->n_types > kMaxTypes)
         return fail(ctx, GDPD_ERR_ARG, "n_types must be in 1..%d", kMaxTypes);
     for (int k = 0; k < 3; ++k)

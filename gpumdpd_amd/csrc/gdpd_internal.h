@@ -7,6 +7,7 @@
 namespace gdpd {
 
 constexpr int kMaxTypes = 12;      // the reference's 256-entry constant tables cap DPD at 12 types
+constexpr int kMaxBeads = (1 << 27) - 1;  // ids share a 32-bit word with the type and a halo flag in LDS
 constexpr int kMaxBondTypes = 16;
 constexpr int kMaxAngleTypes = 16;
 constexpr int kBlock = 256;        // threads per workgroup for every kernel

@@ -10,7 +10,7 @@ struct BrickLaunch {
     int omax;      // max owned beads of a brick
     int ncol;      // max inner columns (bx*by)
     int ntypes;    // max types
-    int qw = 256;  // pair-queue entries per wave
+    int qw = 192;  // pair-queue entries per wave: a flush leaves < 64, one candidate step adds <= 128
     int perbead = 0; // per-bead PE/virial sums (hook path log steps)
     size_t lds_bytes(int precision, int style) const;
 };

@@ -127,8 +127,6 @@ __device__ __forceinline__ void block_sum(double (&acc)[NV], double *red, double
 // batch).
 constexpr double kThermoFix = 4294967296.0;
 
-// owned-index field of a staged bead's tag for a halo bead
-constexpr int kHaloOwner = 0xFFFF;
 
 // face classes of a brick's owned cells (see face_class)
 constexpr int kSigClasses = 32;
@@ -365,13 +363,36 @@ template <typename real> struct Vec4T;
 template <> struct Vec4T<float> { using type = float4; };
 template <> struct Vec4T<double> { using type = double4; };
 
+// Staged bead word (pos.w, as integer bits): id | (type << 27) | (halo << 31).
+// Ids are 1..N with N < 2^27 and types 1..12, so each fits its field.
+constexpr int kIdBits = 27;
+constexpr int kIdMask = (1 << kIdBits) - 1;
+
+__host__ __device__ __forceinline__ int stage_word(int id, int type, bool halo)
+{
+    return id | (type << kIdBits) | (halo ? (int)0x80000000u : 0);
+}
+__device__ __forceinline__ int word_id(int w) { return w & kIdMask; }
+__device__ __forceinline__ int word_type(int w) { return (w >> kIdBits) & 15; }
+__device__ __forceinline__ bool word_halo(int w) { return w < 0; }
+
+// Owned bead info: staged index k (16 bits) and its region cell (5, 5, 6 bits);
+// the host keeps brick edges within those fields.
+__device__ __forceinline__ int owned_info(int k, int rx, int ry, int rz)
+{
+    return k | (rx << 16) | (ry << 21) | (rz << 26);
+}
+
 template <typename real> struct BrickLds {
     using real4 = typename Vec4T<real>::type;
     PairCoef<real> *coef;
-    double *red;
-    real4 *pos, *vel;          // xyz + tag bits (type | owned index << 8), vxyz + id bits
+    double *red;               // block sums (in the queue's space, after the pair phase)
+    real4 *pos;                // staged beads: xyz relative to the brick, w = stage word
+    int *sinfo;                // staged beads: owner index (owned) or slot (halo)
     real *rh;                  // staged density (mDPD force pass)
-    int *c_start, *cinfo, *gstart, *cellpre, *sigcnt, *wbuf, *oslot, *oinfo;
+    int *c_start, *sigcnt, *wbuf, *oinfo;
+    int *cinfo, *gstart, *cellpre; // staging only (in the sums' space)
+    real4 *vown;               // owned beads: vxyz, w = slot bits
     long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
 };
@@ -379,9 +400,13 @@ template <typename real> struct BrickLds {
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
 // LDS layout as byte offsets; pointers are rebuilt inline from the __shared__
-// base so they stay in the LDS address space.
+// base so they stay in the LDS address space.  Halo beads keep only a position
+// and a word (20 bytes in FP32 mode); velocities are staged for owned beads
+// alone, and a pair with a halo bead reads the halo velocity from HBM.  This
+// keeps a 4x4x5-cell brick at rho = 3 under 40 KB, four workgroups per CU.
 struct LdsOffsets {
-    unsigned coef, red, pos, vel, rh, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oslot, oinfo, facc, queue, total;
+    unsigned coef, red, pos, sinfo, rh, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oinfo, vown, facc, queue,
+        total;
 };
 
 template <typename real>
@@ -393,22 +418,24 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     o.field = (unsigned)off;      \
     off = align16(off + (size_t)(bytes));
     GDPD_TAKE(coef, sizeof(PairCoef<real>) * p.ntypes * p.ntypes)
-    GDPD_TAKE(red, sizeof(double) * 64)
     GDPD_TAKE(pos, sizeof(real) * 4 * (p.smax + 1))
-    GDPD_TAKE(vel, sizeof(real) * 4 * p.smax)
+    GDPD_TAKE(sinfo, sizeof(int) * p.smax)
     GDPD_TAKE(rh, sizeof(real) * (p.style == 2 ? p.smax : 1))
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
-    GDPD_TAKE(cinfo, sizeof(int) * p.rmax)
     GDPD_TAKE(sigcnt, sizeof(int) * (kSigClasses + 1))
     GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
-    GDPD_TAKE(oslot, sizeof(int) * p.omax)
     GDPD_TAKE(oinfo, sizeof(int) * p.omax)
-    // gstart and cellpre live only while the brick is staged, before the
-    // per-bead sums are cleared, so they share the sums' space
-    o.gstart = (unsigned)off;
-    o.cellpre = (unsigned)(off + sizeof(int) * p.rmax);
-    GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 2 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 2 * 4 * (size_t)p.rmax)
-    GDPD_TAKE(queue, sizeof(unsigned) * (kBlock / 64) * p.qw)
+    GDPD_TAKE(vown, sizeof(real) * 4 * p.omax)
+    // cinfo, gstart and cellpre live only while the brick is staged, before
+    // the per-bead sums are cleared, so they share the sums' space
+    o.cinfo = (unsigned)off;
+    o.gstart = (unsigned)(off + sizeof(int) * p.rmax);
+    o.cellpre = (unsigned)(off + 2 * sizeof(int) * p.rmax);
+    GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 3 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 3 * 4 * (size_t)p.rmax)
+    // the pair queues; the block sums use the same space after the pair phase
+    o.red = (unsigned)off;
+    GDPD_TAKE(queue, (sizeof(unsigned) * (kBlock / 64) * p.qw > sizeof(double) * 64)
+                         ? sizeof(unsigned) * (kBlock / 64) * p.qw : sizeof(double) * 64)
 #undef GDPD_TAKE
     o.total = (unsigned)off;
     return o;
@@ -422,7 +449,7 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.coef = (PairCoef<real> *)(smem + o.coef);
     L.red = (double *)(smem + o.red);
     L.pos = (real4 *)(smem + o.pos);
-    L.vel = (real4 *)(smem + o.vel);
+    L.sinfo = (int *)(smem + o.sinfo);
     L.rh = (real *)(smem + o.rh);
     L.c_start = (int *)(smem + o.c_start);
     L.cinfo = (int *)(smem + o.cinfo);
@@ -430,8 +457,8 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.cellpre = (int *)(smem + o.cellpre);
     L.sigcnt = (int *)(smem + o.sigcnt);
     L.wbuf = (int *)(smem + o.wbuf);
-    L.oslot = (int *)(smem + o.oslot);
     L.oinfo = (int *)(smem + o.oinfo);
+    L.vown = (real4 *)(smem + o.vown);
     L.facc = (long long *)(smem + o.facc);
     L.queue = (unsigned *)(smem + o.queue);
     return L;
@@ -491,9 +518,17 @@ __device__ __forceinline__ int wrap_code(int c, int n, int &code)
 }
 
 __device__ __forceinline__ float id_to_real(int id, float) { return __int_as_float(id); }
-__device__ __forceinline__ double id_to_real(int id, double) { return __longlong_as_double((long long)id); }
+__device__ __forceinline__ double id_to_real(int id, double) { return __longlong_as_double((long long)(unsigned)id); }
 __device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_longlong(v); }
+
+// A pointer into HBM as a global-address-space pointer: its loads count
+// against vmcnt only, so the brick kernels' waits for LDS results do not also
+// wait for HBM.
+template <typename T> __device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
 
 // Owned beads are numbered cell by cell, grouped by the cell's face class: the
 // brick faces the cell touches (x-low, y-low, y-high, z-low, z-high).  Cells of
@@ -524,7 +559,8 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         const int gy = wrap_code(g.c0[1] - 1 + ry, R.nc[1], wy);
         const int gz = wrap_code(g.c0[2] - 1 + rz, R.nc[2], wz);
         const int gc = (gx * R.nc[1] + gy) * R.nc[2] + gz;
-        const int a = R.cell_start[gc], b = R.cell_start[gc + 1];
+        const auto *cs = gmem(R.cell_start);
+        const int a = cs[gc], b = cs[gc + 1];
         L.cinfo[rc] = rx | (ry << 8) | (rz << 16) | (wx << 24) | (wy << 26) | (wz << 28);
         L.gstart[rc] = a;
         L.c_start[rc + 1] = b - a;
@@ -579,11 +615,17 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     if (S > bp.smax || O > bp.omax) return -1;
 
     const int N = R.N;
-    const double *X = R.x[cur], *V = R.v[cur];
+    const auto *X = gmem(R.x[cur]), *V = gmem(R.v[cur]);
+    const auto *perm = gmem(R.perm);
+    const auto *gid = gmem(R.id);
+    const auto *gtype = gmem(R.type);
+    const auto *grho = gmem(R.rho);
     // four staged beads per thread, 256 apart so each load instruction stays
-    // coalesced; all perm loads and then all state loads are issued together
+    // coalesced; all perm loads and then all state loads are issued together.
+    // Velocities are staged for owned beads only.
     for (int base = t; base < S; base += 4 * kBlock) {
-        int rcs[4], gp[4], sl[4];
+        int rcs[4], gp[4], sl[4], cis[4];
+        bool own[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = base + u * kBlock;
@@ -595,10 +637,14 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             }
             rcs[u] = lo;
             gp[u] = k < S ? L.gstart[lo] + (k - L.c_start[lo]) : -1;
+            const int ci = L.cinfo[lo];
+            const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
+            cis[u] = ci;
+            own[u] = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            int slot = gp[u] >= 0 ? R.perm[gp[u]] : 0;
+            int slot = gp[u] >= 0 ? perm[gp[u]] : 0;
             sl[u] = ((unsigned)slot < (unsigned)N) ? slot : 0;
         }
         double xs[4], ys[4], zs[4], vxs[4], vys[4], vzs[4];
@@ -609,12 +655,13 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             xs[u] = X[sv];
             ys[u] = X[N + sv];
             zs[u] = X[2 * N + sv];
-            tys[u] = R.type[sv];
-            if (need_vel) {
+            tys[u] = gtype[sv];
+            ids[u] = need_vel ? gid[sv] : 0;
+            vxs[u] = vys[u] = vzs[u] = 0.0;
+            if (need_vel && own[u]) {
                 vxs[u] = V[sv];
                 vys[u] = V[N + sv];
                 vzs[u] = V[2 * N + sv];
-                ids[u] = R.id[sv];
             }
         }
 #pragma unroll
@@ -622,40 +669,36 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             const int k = base + u * kBlock;
             if (k >= S) continue;
             const int rc2 = rcs[u];
-            const int ci = L.cinfo[rc2];
+            const int ci = cis[u];
             const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
             const double shx = (double)(((ci >> 24) & 3) - 1) * R.L[0];
             const double shy = (double)(((ci >> 26) & 3) - 1) * R.L[1];
             const double shz = (double)(((ci >> 28) & 3) - 1) * R.L[2];
-            const bool inner = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
-            int ow = kHaloOwner;
-            if (inner) {
-                const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
-                const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
-                ow = o;
-                L.oslot[o] = sl[u];
-                L.oinfo[o] = k | (rc2 << 16);
-            }
             real4 p;
             p.x = (real)((xs[u] + shx) - g.org[0]);
             p.y = (real)((ys[u] + shy) - g.org[1]);
             p.z = (real)((zs[u] + shz) - g.org[2]);
-            // w carries the tag: type | owned index << 8 (kHaloOwner for halo)
-            p.w = id_to_real(tys[u] | (ow << 8), (real)0);
+            p.w = id_to_real(stage_word(ids[u], tys[u], !own[u]), (real)0);
             L.pos[k] = p;
-            if (need_vel) {
+            if (own[u]) {
+                const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
+                const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
+                L.sinfo[k] = o;
+                L.oinfo[o] = owned_info(k, rx, ry, rz);
                 real4 q;
                 q.x = (real)vxs[u];
                 q.y = (real)vys[u];
                 q.z = (real)vzs[u];
-                q.w = id_to_real(ids[u], (real)0);
-                L.vel[k] = q;
+                q.w = id_to_real(sl[u], (real)0);
+                L.vown[o] = q;
+            } else {
+                L.sinfo[k] = sl[u];
             }
-            if (need_rho) L.rh[k] = (real)R.rho[sl[u]];
+            if (need_rho) L.rh[k] = (real)grho[sl[u]];
         }
     }
     for (int k = t; k < R.ntypes * R.ntypes; k += kBlock) L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
-    __syncthreads();  // gstart and cellpre (aliased by facc) are dead from here
+    __syncthreads();  // cinfo, gstart and cellpre (aliased by facc) are dead from here
     for (int k = t; k < 3 * O; k += kBlock) L.facc[k] = 0;
     __syncthreads();
     return O;
@@ -712,23 +755,31 @@ __device__ __forceinline__ int lanes_below(unsigned long long m)
 }
 
 // Thermo partial sums per thread: 0 ke, 1 pair pe, 2 bond pe, 3..8 pressure tensor.
+// Bead i of a queued pair is always owned; j is owned or halo.  A halo j's
+// velocity is read from HBM at its slot and rounded to the working precision
+// exactly as staging rounds an owned bead's.
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real> &L, const unsigned *q, int qn,
-                                           int nt, uint32_t seed_step, bool log, bool perbead,
+                                           int nt, int cur, uint32_t seed_step, bool log, bool perbead,
                                            long long (&acc)[kThermoParts])
 {
     using real4 = typename Vec4T<real>::type;
     const int lane = threadIdx.x & 63;
+    const int N = R.N;
+    // global address space: the loads count against vmcnt only, so waits for
+    // LDS results do not wait for them
+    const __attribute__((address_space(1))) double *V = (const __attribute__((address_space(1))) double *)R.v[cur];
     for (int e = lane; e < qn; e += 64) {
         const unsigned ent = q[e];
         const int li = ent & 0xFFFF, lj = ent >> 16;
         const real4 pi = L.pos[li], pj = L.pos[lj];
-        const int tgi = real_to_id(pi.w), tgj = real_to_id(pj.w);
-        const int oi = tgi >> 8, oj = tgj >> 8;
-        const bool jown = oj != kHaloOwner;
+        const int wi = real_to_id(pi.w), wj = real_to_id(pj.w);
+        const int oi = L.sinfo[li], sj = L.sinfo[lj];  // sj: owner index of an owned j, slot of a halo j
+        const bool jown = !word_halo(wj);
+        const int oj = sj;
         const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const real r2 = dx * dx + dy * dy + dz * dz;
-        const PairCoef<real> &C = L.coef[((tgi & 255) - 1) * nt + ((tgj & 255) - 1)];
+        const PairCoef<real> &C = L.coef[(word_type(wi) - 1) * nt + (word_type(wj) - 1)];
         if (DENSITY) {
             const real r = sqrt(r2);
             // force/pair_mdpd.cu:266-289
@@ -741,15 +792,21 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         }
         // force/pair_dpd.cu: pairs inside rc with r >= 1e-10
         if (r2 > C.rc2 || r2 < (real)1.0e-20) continue;
-        const real4 vi = L.vel[li], vj = L.vel[lj];
+        // the HBM loads of a halo partner's velocity are issued first and used
+        // after the hash, so their latency overlaps it; lanes with an owned
+        // partner load slot 0 (one shared address) and ignore the value
+        const int sv = (!jown && (unsigned)sj < (unsigned)N) ? sj : 0;
+        const double gvx = V[sv], gvy = V[N + sv], gvz = V[2 * N + sv];
         real r, rinv;
         root_and_inverse(r2, r, rinv);
-        const real dot = dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z);
         const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
         const real wr = wc;
-        const int idi = real_to_id(vi.w), idj = real_to_id(vj.w);
+        const int idi = word_id(wi), idj = word_id(wj);
         const uint32_t t1 = (uint32_t)min(idi, idj), t2 = (uint32_t)max(idi, idj);
         const real rnd = saru_uniform<real>(saru_bits(t1, t2, seed_step));
+        const real4 vi = L.vown[oi], vo = L.vown[jown ? oj : oi];
+        const real vjx = jown ? vo.x : (real)gvx, vjy = jown ? vo.y : (real)gvy, vjz = jown ? vo.z : (real)gvz;
+        const real dot = dx * (vi.x - vjx) + dy * (vi.y - vjy) + dz * (vi.z - vjz);
         real wd = (real)0, fpair;
         real rhoi = (real)0, rhoj = (real)0;
         if (STYLE == 2) {
@@ -796,14 +853,14 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             }
             if (perbead) {
                 // hook-path log step: per-bead fixed-point sums in the pe/virial arrays
-                const int N = R.N, si = L.oslot[oi], sj = jown ? L.oslot[oj] : 0;
+                const int si = real_to_id(vi.w), sjs = jown ? real_to_id(L.vown[oj].w) : 0;
                 atomicAdd((unsigned long long *)&R.pe[si], (unsigned long long)to_fix<real>((double)pe_i));
-                if (jown) atomicAdd((unsigned long long *)&R.pe[sj], (unsigned long long)to_fix<real>((double)pe_j));
+                if (jown) atomicAdd((unsigned long long *)&R.pe[sjs], (unsigned long long)to_fix<real>((double)pe_j));
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     const unsigned long long wq = (unsigned long long)to_fix<real>((double)w6[c]);
                     atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + si], wq);
-                    if (jown) atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + sj], wq);
+                    if (jown) atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + sjs], wq);
                 }
             }
         }
@@ -819,8 +876,8 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
 // at a time.
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
-                                            const BrickParams &bp, int O, int nt, real cut2, uint32_t seed_step,
-                                            bool log, long long (&acc)[kThermoParts])
+                                            const BrickParams &bp, int O, int nt, real cut2, int cur,
+                                            uint32_t seed_step, bool log, long long (&acc)[kThermoParts])
 {
     using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -837,10 +894,9 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
         if (valid) {
             const int info = L.oinfo[o];
             li = info & 0xFFFF;
-            const int ci = L.cinfo[info >> 16];
-            rx = ci & 255;
-            ry = (ci >> 8) & 255;
-            rz = (ci >> 16) & 255;
+            rx = (info >> 16) & 31;
+            ry = (info >> 21) & 31;
+            rz = (info >> 26) & 63;
             pi = L.pos[li];
         }
         // Ten candidate segments per bead, each a contiguous LDS range that
@@ -911,7 +967,7 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                     // evaluate whole 64-pair rounds only; the remainder (< 64
                     // entries) moves to the front of the queue
                     const int ne = qn & ~63;
-                    eval_queue<real, STYLE, DENSITY>(R, L, q, ne, nt, seed_step, log, bp.perbead, acc);
+                    eval_queue<real, STYLE, DENSITY>(R, L, q, ne, nt, cur, seed_step, log, bp.perbead, acc);
                     const int rem = qn - ne;
                     if (lane < rem) q[lane] = q[ne + lane];
                     qn = rem;
@@ -924,8 +980,8 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                 bool pa = k < len && (DENSITY ? (r20 < cut2) : (r20 <= cut2));
                 bool pb = k + 1 < len && (DENSITY ? (r21 < cut2) : (r21 <= cut2));
                 if (test_order) {
-                    pa = pa && (la > li || (real_to_id(pj0.w) >> 8) == kHaloOwner);
-                    pb = pb && (la + 1 > li || (real_to_id(pj1.w) >> 8) == kHaloOwner);
+                    pa = pa && (la > li || word_halo(real_to_id(pj0.w)));
+                    pb = pb && (la + 1 > li || word_halo(real_to_id(pj1.w)));
                 }
                 const unsigned long long ma = __ballot(pa);
                 const unsigned long long mb = __ballot(pb);
@@ -937,7 +993,7 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
             }
         }
     }
-    eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, seed_step, log, bp.perbead, acc);
+    eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, cur, seed_step, log, bp.perbead, acc);
     __syncthreads();
 }
 
@@ -966,9 +1022,9 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     long long acc[kThermoParts];
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
-    brick_pairs<real, 2, true>(R, g, L, bp, O, R.ntypes, (real)R.rdmax2, 0u, false, acc);
+    brick_pairs<real, 2, true>(R, g, L, bp, O, R.ntypes, (real)R.rdmax2, cur, 0u, false, acc);
     for (int o = threadIdx.x; o < O; o += kBlock)
-        R.rho[L.oslot[o]] = (double)L.facc[o] * FixScale<real>::inv;
+        R.rho[real_to_id(L.vown[o].w)] = (double)L.facc[o] * FixScale<real>::inv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1001,9 +1057,9 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     long long acc[kThermoParts];
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
-    brick_pairs<real, STYLE, false>(R, g, L, bp, O, R.ntypes, (real)R.rcmax2, seed_step, log, acc);
+    brick_pairs<real, STYLE, false>(R, g, L, bp, O, R.ntypes, (real)R.rcmax2, cur, seed_step, log, acc);
     for (int o = t; o < O; o += kBlock) {
-        const int slot = L.oslot[o];
+        const int slot = real_to_id(L.vown[o].w);
         R.force[slot] = (double)L.facc[3 * o + 0] * FixScale<real>::inv;
         R.force[N + slot] = (double)L.facc[3 * o + 1] * FixScale<real>::inv;
         R.force[2 * N + slot] = (double)L.facc[3 * o + 2] * FixScale<real>::inv;
