@@ -37,6 +37,18 @@ __device__ __forceinline__ int xcd_item(int nitems)
     return x * q + min(x, r) + k;
 }
 
+// A pointer into HBM as a global-address-space pointer: its loads and stores
+// count against vmcnt only, so a kernel's waits for LDS results do not also
+// wait for HBM traffic.
+template <typename T> __device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T> __device__ __forceinline__ __attribute__((address_space(1))) T *gmemw(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 __device__ __forceinline__ int cell_axis(double x, double lo, double inv_cs, int nc)
 {
     int c = (int)((x - lo) * inv_cs);
@@ -267,9 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RepDev *__restrict__ r
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
     if (s >= R.N) return;
-    const int key = R.key[s];
-    const int pos = ((unsigned)key < (unsigned)R.ncell) ? R.cell_start[key] + R.rank[s] : -1;
-    if ((unsigned)pos < (unsigned)R.N) R.perm[pos] = s;
+    const int key = gmem(R.key)[s];
+    const int pos = ((unsigned)key < (unsigned)R.ncell) ? gmem(R.cell_start)[key] + gmem(R.rank)[s] : -1;
+    if ((unsigned)pos < (unsigned)R.N) gmemw(R.perm)[pos] = s;
     else atomicOr(&R.state->flags, kFlagStageOverflow);
 }
 
@@ -522,13 +534,6 @@ __device__ __forceinline__ double id_to_real(int id, double) { return __longlong
 __device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_longlong(v); }
 
-// A pointer into HBM as a global-address-space pointer: its loads count
-// against vmcnt only, so the brick kernels' waits for LDS results do not also
-// wait for HBM.
-template <typename T> __device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p)
-{
-    return (const __attribute__((address_space(1))) T *)p;
-}
 
 // Owned beads are numbered cell by cell, grouped by the cell's face class: the
 // brick faces the cell touches (x-low, y-low, y-high, z-low, z-high).  Cells of
@@ -1058,11 +1063,12 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
     brick_pairs<real, STYLE, false>(R, g, L, bp, O, R.ntypes, (real)R.rcmax2, cur, seed_step, log, acc);
+    auto *F = gmemw(R.force);
     for (int o = t; o < O; o += kBlock) {
         const int slot = real_to_id(L.vown[o].w);
-        R.force[slot] = (double)L.facc[3 * o + 0] * FixScale<real>::inv;
-        R.force[N + slot] = (double)L.facc[3 * o + 1] * FixScale<real>::inv;
-        R.force[2 * N + slot] = (double)L.facc[3 * o + 2] * FixScale<real>::inv;
+        F[slot] = (double)L.facc[3 * o + 0] * FixScale<real>::inv;
+        F[N + slot] = (double)L.facc[3 * o + 1] * FixScale<real>::inv;
+        F[2 * N + slot] = (double)L.facc[3 * o + 2] * FixScale<real>::inv;
         if (log && bp.perbead) {
             R.bpe[slot] = 0.0;
             R.ape[slot] = 0.0;
@@ -1156,18 +1162,21 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
     bool bad = false;
     int my_key = -1;
     if (s < N) {
-        const double *X = R.x[cur], *V = R.v[cur];
-        double fx = R.force[s], fy = R.force[N + s], fz = R.force[2 * N + s];
-        double xg = X[s], yg = X[N + s], zg = X[2 * N + s];
+        const double *X = R.x[cur];
+        const auto *Xg = gmem(R.x[cur]);
+        const auto *V = gmem(R.v[cur]);
+        const auto *F = gmem(R.force);
+        double fx = F[s], fy = F[N + s], fz = F[2 * N + s];
+        double xg = Xg[s], yg = Xg[N + s], zg = Xg[2 * N + s];
         double bpe = 0.0, bw[6] = {0, 0, 0, 0, 0, 0};
         if (TOPO >= 1 && R.nbonds > 0) {
             const int idi = R.id[s];
             const int b1 = R.bond_start[idi];
             for (int bb = R.bond_start[idi - 1]; bb < b1; ++bb) {
                 const int ps = R.slot_of_id[R.bond_partner[bb] - 1];
-                const double ddx = min_image(xg - X[ps], R.L[0], R.hL[0]);
-                const double ddy = min_image(yg - X[N + ps], R.L[1], R.hL[1]);
-                const double ddz = min_image(zg - X[2 * N + ps], R.L[2], R.hL[2]);
+                const double ddx = min_image(xg - Xg[ps], R.L[0], R.hL[0]);
+                const double ddy = min_image(yg - Xg[N + ps], R.L[1], R.hL[1]);
+                const double ddz = min_image(zg - Xg[2 * N + ps], R.L[2], R.hL[2]);
                 const double r = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
                 const int bt = R.bond_type[bb];
                 const double kb = R.bond_k[bt], r0 = R.bond_r0[bt];
@@ -1203,7 +1212,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             R.force[N + s] = fy;
             R.force[2 * N + s] = fz;
         }
-        const int tt = R.type[s] - 1;
+        const int tt = gmem(R.type)[s] - 1;
         const double h = (0.5 * R.dt) * R.minv[tt];
         double vxg = V[s], vyg = V[N + s], vzg = V[2 * N + s];
         // explicit FMAs: the two half-kicks round exactly as two separate
@@ -1230,13 +1239,13 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             vyg = fma(h, fy, vyg);
             vzg = fma(h, fz, vzg);
             int ix, iy, iz;
-            unpack_img(R.img[s], ix, iy, iz);
+            unpack_img(gmem(R.img)[s], ix, iy, iz);
             const int ix0 = ix, iy0 = iy, iz0 = iz;
             bool b2 = false;
             xg = wrap_axis(fma(R.dt, vxg, xg), R.lo[0], R.lo[0] + R.L[0], R.L[0], ix, b2);
             yg = wrap_axis(fma(R.dt, vyg, yg), R.lo[1], R.lo[1] + R.L[1], R.L[1], iy, b2);
             zg = wrap_axis(fma(R.dt, vzg, zg), R.lo[2], R.lo[2] + R.L[2], R.L[2], iz, b2);
-            if (ix != ix0 || iy != iy0 || iz != iz0) R.img[s] = pack_img(ix, iy, iz);
+            if (ix != ix0 || iy != iy0 || iz != iz0) gmemw(R.img)[s] = pack_img(ix, iy, iz);
             if (b2 || !isfinite(vxg + vyg + vzg)) {
                 bad = true;
                 xg = R.lo[0];
@@ -1244,17 +1253,19 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
                 zg = R.lo[2];
             }
             const int key = cell_of(R, xg, yg, zg);
-            double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
+            auto *Xo = gmemw(R.x[1 - cur]);
+            auto *Vo = gmemw(R.v[1 - cur]);
             Xo[s] = xg;
             Xo[N + s] = yg;
             Xo[2 * N + s] = zg;
             Vo[s] = vxg;
             Vo[N + s] = vyg;
             Vo[2 * N + s] = vzg;
-            R.key[s] = key;
+            gmemw(R.key)[s] = key;
             my_key = key;
         } else {
-            double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
+            auto *Xo = gmemw(R.x[1 - cur]);
+            auto *Vo = gmemw(R.v[1 - cur]);
             Xo[s] = xg;
             Xo[N + s] = yg;
             Xo[2 * N + s] = zg;
@@ -1291,7 +1302,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
         for (int e = threadIdx.x; e < kRankSlots; e += kBlock)
             if (hkey[e] >= 0) hbase[e] = atomicAdd(&R.cell_count[hkey[e]], hcnt[e]);
         __syncthreads();
-        if (my_key >= 0) R.rank[s] = hbase[h] + lrank;
+        if (my_key >= 0) gmemw(R.rank)[s] = hbase[h] + lrank;
     }
     if (log) {
         long long *out = red + 48;
