@@ -951,23 +951,10 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
             if (__ballot(len > 0) == 0) continue;
             const int maxlen = wave_max(len);
             const bool test_order = thin && s < 4;
-            // software-pipelined: the next pair of candidates is loaded before
-            // this pair's queue appends, so the LDS latency overlaps the tests
-            // (loads start at most at smax - 1; pos holds smax + 1 entries)
-            real4 nj0, nj1;
-            {
-                const int lc = min(js, bp.smax - 1);
-                nj0 = L.pos[lc];
-                nj1 = L.pos[lc + 1];
-            }
-            for (int k = 0; k < maxlen; k += 2) {
-                const real4 pj0 = nj0, pj1 = nj1;
+            // Candidates k, k + 1 against the queue: the in-range ones are
+            // appended (lanes past their segment are dropped by the k < len test).
+            auto test_pair = [&](const real4 &pj0, const real4 &pj1, int k) {
                 const int la = js + k;
-                {
-                    const int lc = min(la + 2, bp.smax - 1);
-                    nj0 = L.pos[lc];
-                    nj1 = L.pos[lc + 1];
-                }
                 if (qn > qw - 128) {
                     // evaluate whole 64-pair rounds only; the remainder (< 64
                     // entries) moves to the front of the queue
@@ -977,7 +964,6 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                     if (lane < rem) q[lane] = q[ne + lane];
                     qn = rem;
                 }
-                // lanes past their segment are dropped by the k < len test
                 const real dx0 = pi.x - pj0.x, dy0 = pi.y - pj0.y, dz0 = pi.z - pj0.z;
                 const real dx1 = pi.x - pj1.x, dy1 = pi.y - pj1.y, dz1 = pi.z - pj1.z;
                 const real r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
@@ -995,6 +981,30 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                 if (pa) q[qn + lanes_below(ma)] = ent;
                 if (pb) q[qn + na + lanes_below(mb)] = ent + 0x10000u;
                 qn = __builtin_amdgcn_readfirstlane(qn + na + __popcll(mb));
+            };
+            // Software-pipelined with two register sets, so no copies: the next
+            // pair of candidates is loaded before this pair is tested (loads
+            // start at most at smax - 1; pos holds smax + 1 entries).
+            real4 a0, a1, b0, b1;
+            {
+                const int lc = min(js, bp.smax - 1);
+                a0 = L.pos[lc];
+                a1 = L.pos[lc + 1];
+            }
+            for (int k = 0; k < maxlen; k += 4) {
+                {
+                    const int lc = min(js + k + 2, bp.smax - 1);
+                    b0 = L.pos[lc];
+                    b1 = L.pos[lc + 1];
+                }
+                test_pair(a0, a1, k);
+                if (k + 2 >= maxlen) break;
+                {
+                    const int lc = min(js + k + 4, bp.smax - 1);
+                    a0 = L.pos[lc];
+                    a1 = L.pos[lc + 1];
+                }
+                test_pair(b0, b1, k + 2);
             }
         }
     }
