@@ -713,8 +713,6 @@ template <typename real> __device__ __forceinline__ real ratio(real a, real b, r
 template <> __device__ __forceinline__ float ratio<float>(float a, float b, float binv) { return a * binv; }
 template <> __device__ __forceinline__ double ratio<double>(double a, double b, double binv) { return a / b; }
 
-// fixed-point scale of the per-bead sums: 2^32 for FP32 pair arithmetic,
-// 2^40 (|sum| < 8.3e6) for FP64
 // r and 1/r of a pair distance.  FP32 mode uses the hardware reciprocal
 // square root (about 1 ulp, inside the FP32-mode force tolerance); FP64 mode
 // the correctly rounded square root and division.
@@ -730,13 +728,29 @@ __device__ __forceinline__ void root_and_inverse(double r2, double &r, double &r
     rinv = 1.0 / r;
 }
 
+// Fixed-point scale of the per-bead sums: 2^24 for FP32 pair arithmetic (a
+// resolution far below the FP32 rounding of a pair force), 2^40 for FP64.
 template <typename real> struct FixScale;
-template <> struct FixScale<float> { static constexpr double s = 4294967296.0, inv = 1.0 / 4294967296.0; };
+template <> struct FixScale<float> { static constexpr double s = 16777216.0, inv = 1.0 / 16777216.0; };
 template <> struct FixScale<double> { static constexpr double s = 1099511627776.0, inv = 1.0 / 1099511627776.0; };
 
-template <typename real> __device__ __forceinline__ long long to_fix(double v)
+template <typename real> __device__ __forceinline__ long long to_fix(double v);
+
+// v * 2^24 rounded to nearest (ties to even) as an integer, for |v| < 2^27:
+// v * 2^24 + 1.5 * 2^52 lies in [2^52, 2^53), where the doubles are exactly
+// the integers, so the single rounding of the fma is the rounding to an
+// integer and the bits of the sum, less those of 1.5 * 2^52, are that integer.
+// Three instructions (convert, fma, 64-bit add) instead of the eight of a
+// general double-to-int64 conversion.
+template <> __device__ __forceinline__ long long to_fix<float>(double v)
 {
-    return __double2ll_rn(v * FixScale<real>::s);
+    const double y = fma(v, FixScale<float>::s, 6755399441055744.0);
+    return __double_as_longlong(y) - 0x4338000000000000LL;
+}
+
+template <> __device__ __forceinline__ long long to_fix<double>(double v)
+{
+    return __double2ll_rn(v * FixScale<double>::s);
 }
 
 __device__ __forceinline__ void lds_add(long long *p, long long v)
