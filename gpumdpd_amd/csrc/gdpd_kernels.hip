@@ -49,6 +49,7 @@ template <typename T> __device__ __forceinline__ __attribute__((address_space(1)
     return (__attribute__((address_space(1))) T *)p;
 }
 
+
 __device__ __forceinline__ int cell_axis(double x, double lo, double inv_cs, int nc)
 {
     int c = (int)((x - lo) * inv_cs);
@@ -753,6 +754,17 @@ template <> __device__ __forceinline__ long long to_fix<double>(double v)
     return __double2ll_rn(v * FixScale<double>::s);
 }
 
+// The same integer from a value already multiplied by the scale in the working
+// precision (a power-of-two scaling, so exact): a convert and one add in FP32
+// mode.
+template <typename real> __device__ __forceinline__ long long fix_scaled(real vs);
+template <> __device__ __forceinline__ long long fix_scaled<float>(float vs)
+{
+    const double y = (double)vs + 6755399441055744.0;
+    return __double_as_longlong(y) - 0x4338000000000000LL;
+}
+template <> __device__ __forceinline__ long long fix_scaled<double>(double vs) { return __double2ll_rn(vs); }
+
 __device__ __forceinline__ void lds_add(long long *p, long long v)
 {
     atomicAdd((unsigned long long *)p, (unsigned long long)v);
@@ -787,7 +799,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
     const int N = R.N;
     // global address space: the loads count against vmcnt only, so waits for
     // LDS results do not wait for them
-    const __attribute__((address_space(1))) double *V = (const __attribute__((address_space(1))) double *)R.v[cur];
+    const auto *V = gmem(R.v[cur]);
     for (int e = lane; e < qn; e += 64) {
         const unsigned ent = q[e];
         const int li = ent & 0xFFFF, lj = ent >> 16;
@@ -840,8 +852,10 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         fpair -= C.gamma * wr * wr * dot * rinv;
         fpair += C.sig * wr * rnd;
         fpair *= rinv;
-        const real ffx = dx * fpair, ffy = dy * fpair, ffz = dz * fpair;
-        const long long qx = to_fix<real>((double)ffx), qy = to_fix<real>((double)ffy), qz = to_fix<real>((double)ffz);
+        // the force components times the fixed-point scale (exact), straight to fixed point
+        const real fps = fpair * (real)FixScale<real>::s;
+        const long long qx = fix_scaled<real>(dx * fps), qy = fix_scaled<real>(dy * fps),
+                        qz = fix_scaled<real>(dz * fps);
         lds_add(&L.facc[3 * oi + 0], qx);
         lds_add(&L.facc[3 * oi + 1], qy);
         lds_add(&L.facc[3 * oi + 2], qz);
@@ -861,6 +875,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
                 const real ppe = (real)0.5 * C.A * C.rc * wc * wc;
                 pe_i = pe_j = (real)0.5 * ppe;
             }
+            const real ffx = dx * fpair, ffy = dy * fpair, ffz = dz * fpair;
             const real w6[6] = {(real)0.5 * dx * ffx, (real)0.5 * dx * ffy, (real)0.5 * dx * ffz,
                                 (real)0.5 * dy * ffy, (real)0.5 * dy * ffz, (real)0.5 * dz * ffz};
             // per-pair fixed point: the brick sums do not depend on pair order
