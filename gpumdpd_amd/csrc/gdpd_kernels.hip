@@ -914,7 +914,9 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                                             uint32_t seed_step, bool log, long long (&acc)[kThermoParts])
 {
     using real4 = typename Vec4T<real>::type;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int t = threadIdx.x, lane = t & 63;
+    // the wave index is uniform; readfirstlane keeps it (and the queue base) in SGPRs
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int qw = bp.qw;
     unsigned *q = L.queue + w * qw;
     const bool thin = g.ext[2] == 1;
@@ -937,23 +939,31 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
         // holds only partners the bead owns the pair with:
         //   s 0..3  z-runs of the four xy-neighbour columns that precede the
         //           bead's column: whole run when the column is halo, else only
-        //           its halo end cell(s);
+        //           its bottom (rz == 1) or top (rz == RZ - 2) halo cell;
         //   s 4     the bead's own column, bottom halo cell (rz == 1 only);
         //   s 5     the bead's own column after the bead;
         //   s 6..9  z-runs of the four following columns.
-        // Only a brick one cell thick in z (both z-ends halo) keeps its
-        // precedes-or-halo test, in segments 0..3.
-        for (int s = 0; s < 10; ++s) {
+        // A brick one cell thick in z has both z-ends halo: s 0..3 then take
+        // the bottom halo cell and s 10..13 the top one of an owned preceding
+        // column.  A pair of owned beads in two columns is found from the bead
+        // whose column precedes, through its following columns.
+        const int nseg = thin ? 14 : 10;
+        for (int s = 0; s < nseg; ++s) {
             int js = 0, je = 0;
             if (valid) {
-                const int r = s < 5 ? s : s - 1;
+                const int r = s < 5 ? s : (s < 10 ? s - 1 : s - 10);
                 const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
                 const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
                 const int A = L.c_start[cb], D = L.c_start[cb + 3];
                 const bool zlo = (rz == 1), zhi = (rz + 1 == g.RZ - 1);
-                if (s < 4) {
+                if (s < 4 || s >= 10) {
                     const bool xyhalo = cx == 0 || cy == 0 || cy == g.RY - 1;
-                    if (xyhalo || (zlo && zhi)) {
+                    if (s >= 10) {
+                        if (!xyhalo) {
+                            js = L.c_start[cb + 2];
+                            je = D;
+                        }
+                    } else if (xyhalo) {
                         js = A;
                         je = D;
                     } else if (zlo) {
@@ -979,11 +989,11 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
             const int len = je - js;
             if (__ballot(len > 0) == 0) continue;
             const int maxlen = wave_max(len);
-            const bool test_order = thin && s < 4;
+            // queue entry of candidate k: li | (js + k) << 16
+            const unsigned ent0 = (unsigned)li | ((unsigned)js << 16);
             // Candidates k, k + 1 against the queue: the in-range ones are
             // appended (lanes past their segment are dropped by the k < len test).
             auto test_pair = [&](const real4 &pj0, const real4 &pj1, int k) {
-                const int la = js + k;
                 if (qn > qw - 128) {
                     // evaluate whole 64-pair rounds only; the remainder (< 64
                     // entries) moves to the front of the queue
@@ -997,41 +1007,48 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                 const real dx1 = pi.x - pj1.x, dy1 = pi.y - pj1.y, dz1 = pi.z - pj1.z;
                 const real r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
                 const real r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-                bool pa = k < len && (DENSITY ? (r20 < cut2) : (r20 <= cut2));
-                bool pb = k + 1 < len && (DENSITY ? (r21 < cut2) : (r21 <= cut2));
-                if (test_order) {
-                    pa = pa && (la > li || word_halo(real_to_id(pj0.w)));
-                    pb = pb && (la + 1 > li || word_halo(real_to_id(pj1.w)));
-                }
-                const unsigned long long ma = __ballot(pa);
-                const unsigned long long mb = __ballot(pb);
-                const int na = __popcll(ma);
-                const unsigned ent = (unsigned)li | ((unsigned)la << 16);
-                if (pa) q[qn + lanes_below(ma)] = ent;
-                if (pb) q[qn + na + lanes_below(mb)] = ent + 0x10000u;
-                qn = __builtin_amdgcn_readfirstlane(qn + na + __popcll(mb));
+                const bool ina = DENSITY ? (r20 < cut2) : (r20 <= cut2);
+                const bool inb = DENSITY ? (r21 < cut2) : (r21 <= cut2);
+                // Ballots of plain compares combine in SALU (a ballot of a
+                // combined bool is rematerialised per lane); the branches use
+                // the same compares as exec masks.
+                const unsigned long long ma = __builtin_amdgcn_ballot_w64(k < len) & __builtin_amdgcn_ballot_w64(ina);
+                const unsigned long long mb =
+                    __builtin_amdgcn_ballot_w64(k + 1 < len) & __builtin_amdgcn_ballot_w64(inb);
+                // append at the queue's uniform end, in lane order, the second
+                // candidate after the first
+                const unsigned ent = ent0 + ((unsigned)k << 16);
+                unsigned *qa = q + qn;
+                unsigned *qb = qa + __popcll(ma);
+                if (k < len && ina) qa[lanes_below(ma)] = ent;
+                if (k + 1 < len && inb) qb[lanes_below(mb)] = ent + 0x10000u;
+                qn += (int)(__popcll(ma) + __popcll(mb));
             };
             // Software-pipelined with two register sets, so no copies: the next
-            // pair of candidates is loaded before this pair is tested (loads
-            // start at most at smax - 1; pos holds smax + 1 entries).
+            // pair of candidates is loaded before this pair is tested.  Candidate
+            // k + c sits at pj[k + c]; lanes past the staged range read its last
+            // entries instead (loads start at most at smax - 1; pos holds smax + 1
+            // entries).
+            const real4 *pj = L.pos + js;
+            const int lim = bp.smax - 1 - js;
             real4 a0, a1, b0, b1;
             {
-                const int lc = min(js, bp.smax - 1);
-                a0 = L.pos[lc];
-                a1 = L.pos[lc + 1];
+                const int lc = min(0, lim);
+                a0 = pj[lc];
+                a1 = pj[lc + 1];
             }
             for (int k = 0; k < maxlen; k += 4) {
                 {
-                    const int lc = min(js + k + 2, bp.smax - 1);
-                    b0 = L.pos[lc];
-                    b1 = L.pos[lc + 1];
+                    const int lc = min(k + 2, lim);
+                    b0 = pj[lc];
+                    b1 = pj[lc + 1];
                 }
                 test_pair(a0, a1, k);
                 if (k + 2 >= maxlen) break;
                 {
-                    const int lc = min(js + k + 4, bp.smax - 1);
-                    a0 = L.pos[lc];
-                    a1 = L.pos[lc + 1];
+                    const int lc = min(k + 4, lim);
+                    a0 = pj[lc];
+                    a1 = pj[lc + 1];
                 }
                 test_pair(b0, b1, k + 2);
             }
