@@ -376,18 +376,38 @@ template <typename real> struct Vec4T;
 template <> struct Vec4T<float> { using type = float4; };
 template <> struct Vec4T<double> { using type = double4; };
 
-// Staged bead word (pos.w, as integer bits): id | (type << 27) | (halo << 31).
-// Ids are 1..N with N < 2^27 and types 1..12, so each fits its field.
+// Staged bead word (pos.w, as integer bits): id | ((type - 1) << 27) | (halo << 31).
+// Ids are 1..N with N < 2^27 and types 1..12, so each fits its field.  The
+// type is kept zero-based, as it indexes the pair coefficient table.
 constexpr int kIdBits = 27;
 constexpr int kIdMask = (1 << kIdBits) - 1;
 
 __host__ __device__ __forceinline__ int stage_word(int id, int type, bool halo)
 {
-    return id | (type << kIdBits) | (halo ? (int)0x80000000u : 0);
+    return id | ((type - 1) << kIdBits) | (halo ? (int)0x80000000u : 0);
 }
 __device__ __forceinline__ int word_id(int w) { return w & kIdMask; }
-__device__ __forceinline__ int word_type(int w) { return (w >> kIdBits) & 15; }
+__device__ __forceinline__ unsigned word_type0(int w) { return ((unsigned)w >> kIdBits) & 15u; }  // type - 1
 __device__ __forceinline__ bool word_halo(int w) { return w < 0; }
+
+// Byte offsets of LDS elements from 24-bit multiplies (v_mad_u32_u24): LDS
+// indices are small and non-negative, which the compiler cannot see from an
+// int, so a plain index costs a full 32-bit multiply.
+template <typename T> __device__ __forceinline__ T *lds_elem(T *base, unsigned idx, unsigned stride_bytes)
+{
+    return (T *)((char *)base + __umul24(idx, stride_bytes));
+}
+
+// a * b + c on the low 24 bits of a and b (b uniform), one full-rate VALU
+// instruction.  The compiler cannot select it when b comes from another basic
+// block, since the operand's range is not visible there; it then emits a 64-bit
+// multiply-add.
+__device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
+{
+    unsigned r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
 
 // Owned bead info: staged index k (16 bits) and its region cell (5, 5, 6 bits);
 // the host keeps brick edges within those fields.
@@ -408,6 +428,7 @@ template <typename real> struct BrickLds {
     real4 *vown;               // owned beads: vxyz, w = slot bits
     long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
+    unsigned short *cellof;    // staging only: region cell of each staged index (in the queues' space)
 };
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
@@ -419,7 +440,7 @@ __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t
 // keeps a 4x4x5-cell brick at rho = 3 under 40 KB, four workgroups per CU.
 struct LdsOffsets {
     unsigned coef, red, pos, sinfo, rh, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oinfo, vown, facc, queue,
-        total;
+        cellof, total;
 };
 
 template <typename real>
@@ -445,10 +466,16 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     o.gstart = (unsigned)(off + sizeof(int) * p.rmax);
     o.cellpre = (unsigned)(off + 2 * sizeof(int) * p.rmax);
     GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 3 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 3 * 4 * (size_t)p.rmax)
-    // the pair queues; the block sums use the same space after the pair phase
+    // the pair queues; the region cell of each staged index uses the same
+    // space while the brick is staged, the block sums after the pair phase
     o.red = (unsigned)off;
-    GDPD_TAKE(queue, (sizeof(unsigned) * (kBlock / 64) * p.qw > sizeof(double) * 64)
-                         ? sizeof(unsigned) * (kBlock / 64) * p.qw : sizeof(double) * 64)
+    o.cellof = (unsigned)off;
+    {
+        size_t qb = sizeof(unsigned) * (kBlock / 64) * p.qw;
+        if (qb < sizeof(double) * 64) qb = sizeof(double) * 64;
+        if (qb < sizeof(unsigned short) * (size_t)(p.smax + 1)) qb = sizeof(unsigned short) * (size_t)(p.smax + 1);
+        GDPD_TAKE(queue, qb)
+    }
 #undef GDPD_TAKE
     o.total = (unsigned)off;
     return o;
@@ -474,6 +501,7 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.vown = (real4 *)(smem + o.vown);
     L.facc = (long long *)(smem + o.facc);
     L.queue = (unsigned *)(smem + o.queue);
+    L.cellof = (unsigned short *)(smem + o.cellof);
     return L;
 }
 
@@ -595,6 +623,11 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     // owned-cell offsets: bead counts per face class (LDS atomics; the order
     // of cells inside a class is immaterial), then a scan over the classes
     const int nown = g.ext[0] * g.ext[1] * g.ext[2];
+    // region cell of each staged index, read back by the staging loop below
+    for (int rc = t; rc < g.nreg; rc += kBlock) {
+        const int e = min(L.c_start[rc + 1], bp.smax);
+        for (int i = L.c_start[rc]; i < e; ++i) L.cellof[i] = (unsigned short)rc;
+    }
     for (int oc = t; oc < nown; oc += kBlock) {
         int rx, ry, rz;
         split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
@@ -635,12 +668,7 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = base + u * kBlock;
-            int lo = 0, hi = g.nreg;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (L.c_start[mid] <= k) lo = mid;
-                else hi = mid;
-            }
+            const int lo = k < S ? (int)L.cellof[k] : 0;
             rcs[u] = lo;
             gp[u] = k < S ? L.gstart[lo] + (k - L.c_start[lo]) : -1;
             const int ci = L.cinfo[lo];
@@ -810,7 +838,12 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         const int oj = sj;
         const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         const real r2 = dx * dx + dy * dy + dz * dz;
-        const PairCoef<real> &C = L.coef[(word_type(wi) - 1) * nt + (word_type(wj) - 1)];
+        // coefficient row (type_i - 1) * nt, column type_j - 1, as an LDS address
+        using LdsCoef = __attribute__((address_space(3))) const PairCoef<real>;
+        const unsigned cstride = (unsigned)sizeof(PairCoef<real>);
+        const unsigned cbase = (unsigned)(size_t)(LdsCoef *)L.coef;
+        LdsCoef &C = *(LdsCoef *)(size_t)mad_u24(word_type0(wi), (unsigned)nt * cstride,
+                                                 __umul24(word_type0(wj), cstride) + cbase);
         if (DENSITY) {
             const real r = sqrt(r2);
             // force/pair_mdpd.cu:266-289
@@ -856,13 +889,15 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         const real fps = fpair * (real)FixScale<real>::s;
         const long long qx = fix_scaled<real>(dx * fps), qy = fix_scaled<real>(dy * fps),
                         qz = fix_scaled<real>(dz * fps);
-        lds_add(&L.facc[3 * oi + 0], qx);
-        lds_add(&L.facc[3 * oi + 1], qy);
-        lds_add(&L.facc[3 * oi + 2], qz);
+        long long *fi = lds_elem(L.facc, (unsigned)oi, 3u * sizeof(long long));
+        lds_add(fi + 0, qx);
+        lds_add(fi + 1, qy);
+        lds_add(fi + 2, qz);
         if (jown) {
-            lds_sub(&L.facc[3 * oj + 0], qx);
-            lds_sub(&L.facc[3 * oj + 1], qy);
-            lds_sub(&L.facc[3 * oj + 2], qz);
+            long long *fj = lds_elem(L.facc, (unsigned)oj, 3u * sizeof(long long));
+            lds_sub(fj + 0, qx);
+            lds_sub(fj + 1, qy);
+            lds_sub(fj + 2, qz);
         }
         if (log) {
             // per-bead shares as in force/pair_dpd.cu:366-398 and pair_mdpd.cu:479-516
