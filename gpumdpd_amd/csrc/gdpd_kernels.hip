@@ -575,14 +575,41 @@ __device__ __forceinline__ int face_class(const BrickGeom &g, int rx, int ry, in
     return ((rx == 1) << 4) | ((ry == 1) << 3) | ((ry == g.ext[1]) << 2) | ((rz == 1) << 1) | (rz == g.ext[2]);
 }
 
+// Barrier for LDS traffic only: it waits for this wave's LDS operations, not
+// for its global loads, so loads issued before it stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Stage the brick + halo.  Returns (uniformly) the number of owned beads, or
 // -1 when the staged or owned beads exceed the LDS capacity.
+//
+// Three global round trips are on the critical path: the region cells' starts,
+// the permutation, then the bead state.  The pair coefficients are loaded
+// beside the first; the permutation loads are issued before the owned-bead
+// numbering (LDS atomics and scans behind LDS-only barriers) and the state
+// loads in the middle of it, so each overlaps LDS work.
 template <typename real>
 __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
                                            const BrickParams &bp, int cur, bool need_vel, bool need_rho)
 {
     using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x;
+    // pair coefficients, as 32-bit words, held in registers until the LDS write
+    constexpr int kCoefWords = (kMaxTypes * kMaxTypes * (int)sizeof(PairCoef<real>) / 4 + kBlock - 1) / kBlock;
+    const int ncw = R.ntypes * R.ntypes * (int)(sizeof(PairCoef<real>) / 4);
+    int cw[kCoefWords];
+    {
+        const auto *cg = gmem((const int *)R.coef);
+#pragma unroll
+        for (int u = 0; u < kCoefWords; ++u) {
+            const int k = t + u * kBlock;
+            cw[u] = k < ncw ? cg[k] : 0;
+        }
+    }
     const float inv_rz = 1.0f / (float)g.RZ, inv_ryz = 1.0f / (float)(g.RY * g.RZ);
     const float inv_ez = 1.0f / (float)g.ext[2], inv_eyz = 1.0f / (float)(g.ext[1] * g.ext[2]);
     for (int rc = t; rc < g.nreg; rc += kBlock) {
@@ -598,6 +625,11 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         L.cinfo[rc] = rx | (ry << 8) | (rz << 16) | (wx << 24) | (wy << 26) | (wz << 28);
         L.gstart[rc] = a;
         L.c_start[rc + 1] = b - a;
+    }
+#pragma unroll
+    for (int u = 0; u < kCoefWords; ++u) {
+        const int k = t + u * kBlock;
+        if (k < ncw) ((int *)L.coef)[k] = cw[u];
     }
     __syncthreads();
     {
@@ -620,38 +652,14 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         if (t <= kSigClasses) L.sigcnt[t] = 0;
     }
     __syncthreads();
-    // owned-cell offsets: bead counts per face class (LDS atomics; the order
-    // of cells inside a class is immaterial), then a scan over the classes
-    const int nown = g.ext[0] * g.ext[1] * g.ext[2];
-    // region cell of each staged index, read back by the staging loop below
+    const int S = L.c_start[g.nreg];
+    const int Sc = min(S, bp.smax);  // indices past the capacity are not staged
+    // region cell of each staged index
     for (int rc = t; rc < g.nreg; rc += kBlock) {
         const int e = min(L.c_start[rc + 1], bp.smax);
         for (int i = L.c_start[rc]; i < e; ++i) L.cellof[i] = (unsigned short)rc;
     }
-    for (int oc = t; oc < nown; oc += kBlock) {
-        int rx, ry, rz;
-        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
-        ++rx, ++ry, ++rz;
-        const int rc = (rx * g.RY + ry) * g.RZ + rz;
-        L.cellpre[oc] = atomicAdd(&L.sigcnt[face_class(g, rx, ry, rz)], L.c_start[rc + 1] - L.c_start[rc]);
-    }
-    __syncthreads();
-    if (t < 64) {
-        const int v = t < kSigClasses ? L.sigcnt[t] : 0;
-        const int inc = wave_incl_scan(v);
-        if (t < kSigClasses) L.sigcnt[t] = inc - v;
-        if (t == kSigClasses - 1) L.sigcnt[kSigClasses] = inc;
-    }
-    __syncthreads();
-    for (int oc = t; oc < nown; oc += kBlock) {
-        int rx, ry, rz;
-        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
-        L.cellpre[oc] += L.sigcnt[face_class(g, rx + 1, ry + 1, rz + 1)];
-    }
-    __syncthreads();
-    const int S = L.c_start[g.nreg];
-    const int O = L.sigcnt[kSigClasses];
-    if (S > bp.smax || O > bp.omax) return -1;
+    lds_barrier();
 
     const int N = R.N;
     const auto *X = gmem(R.x[cur]), *V = gmem(R.v[cur]);
@@ -659,32 +667,33 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     const auto *gid = gmem(R.id);
     const auto *gtype = gmem(R.type);
     const auto *grho = gmem(R.rho);
-    // four staged beads per thread, 256 apart so each load instruction stays
-    // coalesced; all perm loads and then all state loads are issued together.
-    // Velocities are staged for owned beads only.
-    for (int base = t; base < S; base += 4 * kBlock) {
-        int rcs[4], gp[4], sl[4], cis[4];
-        bool own[4];
+    // Four staged beads per thread, 256 apart so each load instruction stays
+    // coalesced.  lookup() finds their region cells and issues the permutation
+    // loads; fetch() issues the state loads.  Velocities are staged for owned
+    // beads only.
+    int rcs[4], cis[4], sl[4];
+    bool own[4];
+    double xs[4], ys[4], zs[4], vxs[4], vys[4], vzs[4];
+    int ids[4], tys[4];
+    auto lookup = [&](int base) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = base + u * kBlock;
-            const int lo = k < S ? (int)L.cellof[k] : 0;
+            const bool in = k < Sc;
+            const int lo = in ? (int)L.cellof[k] : 0;
             rcs[u] = lo;
-            gp[u] = k < S ? L.gstart[lo] + (k - L.c_start[lo]) : -1;
+            const int gp = in ? L.gstart[lo] + (k - L.c_start[lo]) : -1;
             const int ci = L.cinfo[lo];
             const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
             cis[u] = ci;
             own[u] = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
+            sl[u] = gp >= 0 ? perm[gp] : 0;
         }
+    };
+    auto fetch = [&]() {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            int slot = gp[u] >= 0 ? perm[gp[u]] : 0;
-            sl[u] = ((unsigned)slot < (unsigned)N) ? slot : 0;
-        }
-        double xs[4], ys[4], zs[4], vxs[4], vys[4], vzs[4];
-        int ids[4], tys[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
+            sl[u] = ((unsigned)sl[u] < (unsigned)N) ? sl[u] : 0;
             const int sv = sl[u];
             xs[u] = X[sv];
             ys[u] = X[N + sv];
@@ -697,6 +706,42 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
                 vys[u] = V[N + sv];
                 vzs[u] = V[2 * N + sv];
             }
+        }
+    };
+    lookup(t);
+
+    // owned-cell offsets: bead counts per face class (LDS atomics; the order
+    // of cells inside a class is immaterial), then a scan over the classes
+    const int nown = g.ext[0] * g.ext[1] * g.ext[2];
+    for (int oc = t; oc < nown; oc += kBlock) {
+        int rx, ry, rz;
+        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
+        ++rx, ++ry, ++rz;
+        const int rc = (rx * g.RY + ry) * g.RZ + rz;
+        L.cellpre[oc] = atomicAdd(&L.sigcnt[face_class(g, rx, ry, rz)], L.c_start[rc + 1] - L.c_start[rc]);
+    }
+    lds_barrier();
+    if (t < 64) {
+        const int v = t < kSigClasses ? L.sigcnt[t] : 0;
+        const int inc = wave_incl_scan(v);
+        if (t < kSigClasses) L.sigcnt[t] = inc - v;
+        if (t == kSigClasses - 1) L.sigcnt[kSigClasses] = inc;
+    }
+    fetch();
+    lds_barrier();
+    for (int oc = t; oc < nown; oc += kBlock) {
+        int rx, ry, rz;
+        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
+        L.cellpre[oc] += L.sigcnt[face_class(g, rx + 1, ry + 1, rz + 1)];
+    }
+    lds_barrier();
+    const int O = L.sigcnt[kSigClasses];
+    if (S > bp.smax || O > bp.omax) return -1;
+
+    for (int base = t; base < S; base += 4 * kBlock) {
+        if (base != t) {
+            lookup(base);
+            fetch();
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -731,7 +776,6 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             if (need_rho) L.rh[k] = (real)grho[sl[u]];
         }
     }
-    for (int k = t; k < R.ntypes * R.ntypes; k += kBlock) L.coef[k] = ((const PairCoef<real> *)R.coef)[k];
     __syncthreads();  // cinfo, gstart and cellpre (aliased by facc) are dead from here
     for (int k = t; k < 3 * O; k += kBlock) L.facc[k] = 0;
     __syncthreads();
@@ -936,8 +980,10 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
     }
 }
 
-// Generation + evaluation.  Wave w takes owned beads in chunks of 64 (chunks
-// w, w+4, ...).  Each lane scans its bead's 27 neighbour cells as contiguous
+// Generation + evaluation.  The owned beads are split into 4 * nr equal chunks
+// of at most 64 (nr = ceil(O / 256) rounds), and wave w takes chunks w, w + 4,
+// ...: equal chunks keep the four waves' pair phases about equally long, so
+// few wait at the closing barrier.  Each lane scans its bead's 27 neighbour cells as contiguous
 // LDS segments that hold only the partners j the bead owns the pair with: j
 // after the bead in LDS order, or a halo bead.  A pair of owned beads is thus
 // found once and a pair with a halo bead once per brick.  In-range candidates
@@ -956,9 +1002,12 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
     unsigned *q = L.queue + w * qw;
     const bool thin = g.ext[2] == 1;
     int qn = 0;
-    for (int chunk = w; chunk * 64 < O; chunk += kBlock / 64) {
-        const int o = chunk * 64 + lane;
-        const bool valid = o < O;
+    const int nwave = kBlock / 64;
+    const int nchunk = nwave * max(1, (O + kBlock - 1) / kBlock);
+    const int per = (O + nchunk - 1) / nchunk;  // <= 64; 0 when the brick owns no bead
+    for (int chunk = w; chunk < nchunk; chunk += nwave) {
+        const int o = chunk * per + lane;
+        const bool valid = lane < per && o < O;
         int li = 0, rx = 1, ry = 1, rz = 1;
         real4 pi;
         pi.x = pi.y = pi.z = pi.w = (real)0;
