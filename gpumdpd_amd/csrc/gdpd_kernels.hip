@@ -848,9 +848,6 @@ __device__ __forceinline__ void lds_sub(long long *p, long long v)
                            __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// wavefront maximum (DPP row/wave shifts, result read from one lane)
-extern "C" __device__ int __ockl_wfred_max_i32(int);
-__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 
 __device__ __forceinline__ int lanes_below(unsigned long long m)
 {
@@ -1071,8 +1068,7 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                 }
             }
             const int len = je - js;
-            if (__ballot(len > 0) == 0) continue;
-            const int maxlen = wave_max(len);
+            if (__builtin_amdgcn_ballot_w64(len > 0) == 0) continue;
             // queue entry of candidate k: li | (js + k) << 16
             const unsigned ent0 = (unsigned)li | ((unsigned)js << 16);
             // Candidates k, k + 1 against the queue: the in-range ones are
@@ -1121,20 +1117,23 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
                 a0 = pj[lc];
                 a1 = pj[lc + 1];
             }
-            for (int k = 0; k < maxlen; k += 4) {
+            // the loop ends when no lane has a candidate left (a ballot of the
+            // next step's compare, in place of a wave maximum of len)
+            for (int k = 0;; k += 4) {
                 {
                     const int lc = min(k + 2, lim);
                     b0 = pj[lc];
                     b1 = pj[lc + 1];
                 }
                 test_pair(a0, a1, k);
-                if (k + 2 >= maxlen) break;
+                if (__builtin_amdgcn_ballot_w64(k + 2 < len) == 0) break;
                 {
                     const int lc = min(k + 4, lim);
                     a0 = pj[lc];
                     a1 = pj[lc + 1];
                 }
                 test_pair(b0, b1, k + 2);
+                if (__builtin_amdgcn_ballot_w64(k + 4 < len) == 0) break;
             }
         }
     }
