@@ -980,12 +980,12 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
 // Generation + evaluation.  The owned beads are split into 4 * nr equal chunks
 // of at most 64 (nr = ceil(O / 256) rounds), and wave w takes chunks w, w + 4,
 // ...: equal chunks keep the four waves' pair phases about equally long, so
-// few wait at the closing barrier.  Each lane scans its bead's 27 neighbour cells as contiguous
-// LDS segments that hold only the partners j the bead owns the pair with: j
-// after the bead in LDS order, or a halo bead.  A pair of owned beads is thus
-// found once and a pair with a halo bead once per brick.  In-range candidates
-// are appended to a per-wave queue (ballot + mbcnt) that is evaluated 64 pairs
-// at a time.
+// few wait at the closing barrier.  Each lane scans its bead's 27 neighbour
+// cells as contiguous LDS segments that hold only the partners j the bead owns
+// the pair with: j after the bead in LDS order, or a halo bead.  A pair of
+// owned beads is thus found once and a pair with a halo bead once per brick.
+// In-range candidates are appended to a per-wave queue (ballot + mbcnt) that
+// is evaluated 64 pairs at a time.
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
                                             const BrickParams &bp, int O, int nt, real cut2, int cur,
