@@ -5,8 +5,11 @@
  * every multiply and add rounds exactly as written.
  *
  * What pins each function (checked in tests/test_oracle_*.py):
- *   orc_saru_*        committed Saru table cross-checked against an independent
- *                     numpy uint32 restatement; range/mean/variance/decorrelation.
+ *   orc_saru_*        TEA stage equal to a textbook TEA (Wheeler-Needham 1994) that
+ *                     reproduces the published zero-key known answer; pre-mix and
+ *                     avalanche: committed table cross-checked against an
+ *                     independent numpy uint32 restatement (no second published
+ *                     source); range/mean/variance/decorrelation.
  *   orc_neigh_build   set equality with orc_neigh_brute (O(N^2)) incl. clamp and
  *                     box-edge cases; brute force is the definition.
  *   orc_neigh_check   exact threshold cases at skin/2.
@@ -14,7 +17,11 @@
  *                     separately), Newton's third law, zero beyond rc,
  *                     gamma=0 energy conservation of a short NVE run.
  *   orc_mdpd_*        isolated-pair density closed form; 3-bead density sums;
- *                     2-bead force closed form; third law.
+ *                     2-bead force closed form; third law; force = -grad of
+ *                     Warren's many-body potential; per-bead PE = half of each
+ *                     pair energy + (pi rd^4/30) B rho_i^2 (hand-worked line of
+ *                     4 beads with unequal densities, brute force on an uneven
+ *                     cluster) and -grad(sum PE) = force.
  *   orc_bond_force    closed form F = -2k(r-r0) r_hat, E = k(r-r0)^2; a 2-bead
  *                     bonded NVE run conserves energy.
  *   orc_angle_force   F = -grad of E = k (theta - theta0)^2 by central differences,
@@ -41,7 +48,9 @@
 
 static inline uint32_t sar32(uint32_t v, int n) { return (uint32_t)((int32_t)v >> n); }
 
-uint32_t orc_saru_u32(uint32_t a, uint32_t b, uint32_t seed, int32_t timestep)
+/* Pre-mix of the seeds into the TEA block (pair_dpd.cu:213-226). */
+static inline void saru_premix(uint32_t a, uint32_t b, uint32_t seed, int32_t timestep, uint32_t *st_out,
+                               uint32_t *ws_out)
 {
     uint32_t c = seed + (uint32_t)timestep;
     c ^= (a << 7) ^ (b >> 6);
@@ -58,15 +67,49 @@ uint32_t orc_saru_u32(uint32_t a, uint32_t b, uint32_t seed, int32_t timestep)
     uint32_t ws = (st + b) ^ sar32(st, 8);
     st = st + ws * (ws ^ 0xdddf97f5u);
     ws = 0xABCB96F7u + (ws >> 1);
+    *st_out = st;
+    *ws_out = ws;
+}
 
+/* The 8 TEA cycles on the block (st, ws) (pair_dpd.cu:227-232). */
+static inline void saru_tea8(uint32_t *st_io, uint32_t *ws_io)
+{
+    uint32_t st = *st_io, ws = *ws_io;
     uint32_t sum = 0;
     for (int round = 0; round < 8; ++round) {
         sum += 0x9e3779b9u;
         st += ((ws << 4) + 0xA341316Cu) ^ (ws + sum) ^ ((ws >> 5) + 0xC8013EA4u);
         ws += ((st << 4) + 0xAD90777Du) ^ (st + sum) ^ ((st >> 5) + 0x7E95761Eu);
     }
+    *st_io = st;
+    *ws_io = ws;
+}
+
+/* Output avalanche (pair_dpd.cu:233-234). */
+static inline uint32_t saru_finish(uint32_t st, uint32_t ws)
+{
     uint32_t v = (st ^ (st >> 26)) + ws;
     return (v ^ (v >> 20)) * 0x6957f5a7u;
+}
+
+uint32_t orc_saru_u32(uint32_t a, uint32_t b, uint32_t seed, int32_t timestep)
+{
+    uint32_t st, ws;
+    saru_premix(a, b, seed, timestep, &st, &ws);
+    saru_tea8(&st, &ws);
+    return saru_finish(st, ws);
+}
+
+void orc_saru_stages(uint32_t a, uint32_t b, uint32_t seed, int32_t timestep, uint32_t out[5])
+{
+    uint32_t st, ws;
+    saru_premix(a, b, seed, timestep, &st, &ws);
+    out[0] = st;
+    out[1] = ws;
+    saru_tea8(&st, &ws);
+    out[2] = st;
+    out[3] = ws;
+    out[4] = saru_finish(st, ws);
 }
 
 float orc_saru_rand_f32(uint32_t a, uint32_t b, uint32_t seed, int32_t timestep)

@@ -36,6 +36,10 @@ float    orc_saru_rand_f32(uint32_t seed1, uint32_t seed2, uint32_t seed, int32_
 double   orc_saru_rand_f64(uint32_t seed1, uint32_t seed2, uint32_t seed, int32_t timestep);
 void     orc_saru_batch(int n, const uint32_t *tag1, const uint32_t *tag2, uint32_t seed,
                         const int32_t *steps, uint32_t *s_out, float *r32_out, double *r64_out);
+/* The hash's three stages: out[0..1] = the TEA block (st, ws) after the seed
+   pre-mix (pair_dpd.cu:213-226), out[2..3] = the block after the 8 TEA cycles
+   (:227-232), out[4] = the finished 32-bit value (:233-234) = orc_saru_u32. */
+void     orc_saru_stages(uint32_t seed1, uint32_t seed2, uint32_t seed, int32_t timestep, uint32_t out[5]);
 
 /* ---- packed pair-coefficient index (force/pair_dpd.cu:134, 0-based) ---- */
 int orc_pair_index(int ti, int tj, int ntypes);

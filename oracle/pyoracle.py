@@ -55,6 +55,7 @@ def lib():
         L.orc_saru_rand_f64.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32]
         L.orc_saru_batch.argtypes = [C.c_int, _PU, _PU, C.c_uint32, C.POINTER(C.c_int32), _PU,
                                      C.POINTER(C.c_float), _PD]
+        L.orc_saru_stages.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _PU]
         L.orc_pair_index.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_neigh_build.argtypes = [C.c_int, _PD, C.POINTER(Box), C.c_double, C.c_int, _PI, _PI]
         L.orc_neigh_brute.argtypes = [C.c_int, _PD, C.POINTER(Box), C.c_double, C.c_int, _PI, _PI]
@@ -101,6 +102,14 @@ def saru_batch(t1, t2, seed, steps):
     lib().orc_saru_batch(n, p(t1, C.c_uint32), p(t2, C.c_uint32), int(seed) & 0xFFFFFFFF, p(steps, C.c_int32),
                          p(s, C.c_uint32), p(r32, C.c_float), p(r64))
     return s, r32, r64
+
+
+def saru_stages(t1, t2, seed, step):
+    """(premixed st, ws, st and ws after the 8 TEA cycles, finished value) of one call."""
+    out = (C.c_uint32 * 5)()
+    lib().orc_saru_stages(int(t1) & 0xFFFFFFFF, int(t2) & 0xFFFFFFFF, int(seed) & 0xFFFFFFFF, int(step),
+                          C.cast(out, _PU))
+    return tuple(int(v) for v in out)
 
 
 def n_max_neigh(global_cut, skin, max_rho):
