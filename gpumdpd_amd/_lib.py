@@ -20,6 +20,7 @@ COMPUTE_KINDS = {"temp": 1, "msd": 2, "rho": 3, "tf": 4, "wall_contact": 5}
 HOOKS = {"initial_integrate": 0, "post_integrate": 1, "post_force": 2, "final_integrate": 3, "end_of_step": 4}
 MAX_FIXES = MAX_COMPUTES = 8
 RECORD_LEN, RECORD_CAP = 6, 1024
+STEP_TS_CAP = 256
 
 
 class SystemDesc(C.Structure):
@@ -92,6 +93,8 @@ PROTOTYPES = {
     "gdpd_kernel_timing": (_I, [_P, _I]),
     "gdpd_kernel_timing_get": (_I, [_P, _PD, C.POINTER(C.c_int64), _PD]),
     "gdpd_sync": (_I, [_P]),
+    "gdpd_set_capacity": (_I, [_P, _I, _I, _I]),
+    "gdpd_step_times": (_I, [_P, _PI, _I, C.POINTER(C.c_uint64), C.POINTER(C.c_int64), _PD]),
     "gdpd_saru_eval": (_I, [_P, _I, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32,
                             C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.POINTER(C.c_float), _PD]),
 }

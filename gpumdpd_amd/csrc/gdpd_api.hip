@@ -72,6 +72,9 @@ struct Sys {
     int resort_every = 0, graph_steps = 0;
     int steps_since_resort = 0;
     bool midstep = false;       // between a zero_step/advance and finish
+    long long step_host = 0;    // the device step counter once the enqueued work is done
+    int cap_s = 0, cap_o = 0;   // LDS capacities set by the caller (0: from the state)
+    int regrows = 0;            // LDS capacity increases after a staging overflow
     Tables tab;                 // single-system work tables
     Graph graph;                // single-system step graph
     // reference neighbour list
@@ -109,6 +112,15 @@ struct gdpd_ctx {
     Tables batch;
     Graph batch_graph;
     int max_lds = 65536;
+    // systems advanced by enqueued work whose halt flags have not been read yet
+    std::vector<int> pending;
+    // gather buffers (states, step-time rings) of listed systems
+    int *d_ids = nullptr;
+    RepState *d_states = nullptr;
+    unsigned long long *d_ts = nullptr;
+    long long *d_tsteps = nullptr;
+    int gather_cap = 0;
+    double tick_ns = 10.0;  // device wall clock period
 };
 
 namespace {
@@ -409,6 +421,10 @@ int check_flags(gdpd_ctx *ctx, Sys *s)
                     s->h.smax, s->h.omax);
     if (st.flags & kFlagNonFinite)
         return fail(ctx, GDPD_ERR_NAN, "non-finite or runaway bead coordinates");
+    if (st.flags & kFlagImageOverflow)
+        return fail(ctx, GDPD_ERR_OVERFLOW, "a periodic image count left +-%d box crossings", kImgMax);
+    if (st.flags & kFlagScatter)
+        return fail(ctx, GDPD_ERR_HIP, "internal error: a binned bead fell outside the bead range");
     return GDPD_OK;
 }
 
@@ -444,6 +460,54 @@ int choose_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cell_start)
     h.omax = std::min(65000, std::max(32, (int)std::ceil(maxown * 1.15) + 16));
     (void)ctx;
     return GDPD_OK;
+}
+
+// Capacities for the binned state cs_host, at least smin / omin, then the brick
+// shrunk (longest edge first) until its LDS layout fits the device.
+int fit_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cs_host, int smin, int omin)
+{
+    RepDev &h = s->h;
+    int rc;
+    auto choose = [&]() {
+        int r = choose_capacities(ctx, s, cs_host);
+        h.smax = std::max(h.smax, smin);
+        h.omax = std::max(h.omax, omin);
+        return r;
+    };
+    if ((rc = choose())) return rc;
+    for (int guard = 0; guard < 16; ++guard) {
+        BrickLaunch b = brick_params_of(s);
+        if (b.lds_bytes(h.precision, s->style) <= (size_t)ctx->max_lds) return GDPD_OK;
+        int dmax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (h.bdim[d] > h.bdim[dmax]) dmax = d;
+        if (h.bdim[dmax] <= 1) return fail(ctx, GDPD_ERR_OVERFLOW, "bead density too high for LDS staging");
+        h.bdim[dmax]--;
+        for (int d = 0; d < 3; ++d) h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
+        h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
+        (void)hipFree(h.thermo_part);
+        s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), (void *)h.thermo_part));
+        if ((rc = dalloc(ctx, s, &h.thermo_part, (size_t)h.nbrick * kThermoParts))) return rc;
+        if ((rc = choose())) return rc;
+    }
+    return fail(ctx, GDPD_ERR_OVERFLOW, "bead density too high for LDS staging");
+}
+
+// capacities the caller set (gdpd_set_capacity) in place of the automatic ones
+int apply_set_capacity(gdpd_ctx *ctx, Sys *s)
+{
+    RepDev &h = s->h;
+    if (s->cap_s > 0) h.smax = s->cap_s;
+    if (s->cap_o > 0) h.omax = s->cap_o;
+    if (brick_params_of(s).lds_bytes(h.precision, s->style) > (size_t)ctx->max_lds)
+        return fail(ctx, GDPD_ERR_ARG, "capacities smax=%d, omax=%d do not fit in LDS with a %dx%dx%d brick", h.smax,
+                    h.omax, h.bdim[0], h.bdim[1], h.bdim[2]);
+    return GDPD_OK;
+}
+
+int sys_index(gdpd_ctx *ctx, const Sys *s)
+{
+    return (int)(std::find(ctx->sys.begin(), ctx->sys.end(), s) - ctx->sys.begin());
 }
 
 int prepare(gdpd_ctx *ctx, Sys *s)
@@ -525,23 +589,8 @@ int prepare(gdpd_ctx *ctx, Sys *s)
     std::vector<int> cs_host((size_t)h.ncell + 1);
     HIPCHK(hipMemcpyAsync(cs_host.data(), h.cell_start, sizeof(int) * cs_host.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if ((rc = choose_capacities(ctx, s, cs_host))) return rc;
-    // shrink the brick until the LDS layout fits
-    for (int guard = 0; guard < 16; ++guard) {
-        BrickLaunch b = brick_params_of(s);
-        if (b.lds_bytes(h.precision, s->style) <= (size_t)ctx->max_lds) break;
-        int dmax = 0;
-        for (int d = 1; d < 3; ++d)
-            if (h.bdim[d] > h.bdim[dmax]) dmax = d;
-        if (h.bdim[dmax] <= 1) return fail(ctx, GDPD_ERR_OVERFLOW, "bead density too high for LDS staging");
-        h.bdim[dmax]--;
-        for (int d = 0; d < 3; ++d) h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
-        h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
-        (void)hipFree(h.thermo_part);
-        s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), (void *)h.thermo_part));
-        if ((rc = dalloc(ctx, s, &h.thermo_part, (size_t)h.nbrick * kThermoParts))) return rc;
-        if ((rc = choose_capacities(ctx, s, cs_host))) return rc;
-    }
+    if ((rc = fit_capacities(ctx, s, cs_host, 0, 0))) return rc;
+    if ((rc = apply_set_capacity(ctx, s))) return rc;
     if ((rc = push_desc(ctx, id))) return rc;
     if ((rc = build_tables(ctx, {id}, s->tab))) return rc;
     s->cells_valid = true;
@@ -566,6 +615,7 @@ int set_state_step(gdpd_ctx *ctx, Sys *s, long long step, bool reset_rows)
     int rc = read_state(ctx, s, &st);
     if (rc) return rc;
     st.step = step;
+    s->step_host = step;
     if (reset_rows) st.log_rows = 0;
     HIPCHK(hipMemcpyAsync(s->h.state, &st, sizeof(RepState), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -830,6 +880,152 @@ int fix_zero_step(gdpd_ctx *ctx, Sys *s, bool next_initial)
     s->fix_step = 0;
     return fix_iteration(ctx, s, false, false, next_initial);
 }
+
+// ---- staging overflow: hold, grow, resume --------------------------------------
+//
+// A brick whose beads exceed its LDS capacity halts its system (kFlagHalt):
+// every later step kernel leaves that system's state alone, so it stays at the
+// last completed step while the rest of a batch goes on.  When the host next
+// looks (settle), it sizes the capacities again from the held state's cell
+// lists, clears the flag and runs the missing steps for that system alone.  A
+// system's arithmetic depends only on its own state and brick shape, so the
+// result equals a run that never overflowed, bit for bit, as long as the brick
+// shape itself still fits in LDS.
+
+// scalar states of the systems ids: one gather launch and one copy
+int read_states(gdpd_ctx *ctx, const std::vector<int> &ids, std::vector<RepState> &out)
+{
+    const int n = (int)ids.size();
+    out.resize(n);
+    if (!n) return GDPD_OK;
+    if (n > ctx->gather_cap) {
+        int cap = std::max(64, ctx->gather_cap);
+        while (cap < n) cap *= 2;
+        void *old[] = {ctx->d_ids, ctx->d_states, ctx->d_ts, ctx->d_tsteps};
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (void *q : old)
+            if (q) (void)hipFree(q);
+        ctx->d_ids = nullptr;
+        ctx->d_states = nullptr;
+        ctx->d_ts = nullptr;
+        ctx->d_tsteps = nullptr;
+        ctx->gather_cap = 0;
+        HIPCHK(hipMalloc((void **)&ctx->d_ids, sizeof(int) * cap));
+        HIPCHK(hipMalloc((void **)&ctx->d_states, sizeof(RepState) * cap));
+        HIPCHK(hipMalloc((void **)&ctx->d_ts, sizeof(unsigned long long) * kStepTsCap * cap));
+        HIPCHK(hipMalloc((void **)&ctx->d_tsteps, sizeof(long long) * cap));
+        ctx->gather_cap = cap;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->d_ids, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_gather_state(ctx->d_reps, ctx->d_ids, n, ctx->d_states, ctx->stream);
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out.data(), ctx->d_states, sizeof(RepState) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+// New LDS capacities from the held state's cell lists (the brick shrinks only
+// when they no longer fit); tables and graphs are rebuilt, the halt cleared.
+int regrow(gdpd_ctx *ctx, Sys *s)
+{
+    RepDev &h = s->h;
+    std::vector<int> cs((size_t)h.ncell + 1);
+    HIPCHK(hipMemcpyAsync(cs.data(), h.cell_start, sizeof(int) * cs.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int s0 = h.smax, o0 = h.omax, b0[3] = {h.bdim[0], h.bdim[1], h.bdim[2]};
+    int rc;
+    if ((rc = fit_capacities(ctx, s, cs, 0, 0))) return rc;
+    const bool same_brick = h.bdim[0] == b0[0] && h.bdim[1] == b0[1] && h.bdim[2] == b0[2];
+    if (same_brick && h.smax <= s0 && h.omax <= o0)
+        return fail(ctx, GDPD_ERR_OVERFLOW, "a brick holds more beads than the largest LDS capacity (smax=%d, omax=%d)",
+                    h.smax, h.omax);
+    s->regrows++;
+    const int id = sys_index(ctx, s);
+    if ((rc = push_desc(ctx, id))) return rc;
+    if ((rc = build_tables(ctx, {id}, s->tab))) return rc;
+    s->graph.release();
+    ctx->batch_graph.release();
+    ctx->batch.release();
+    RepState st;
+    if ((rc = read_state(ctx, s, &st))) return rc;
+    st.flags &= ~kFlagHalt;
+    HIPCHK(hipMemcpyAsync(h.state, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return GDPD_OK;
+}
+
+// The steps a system held at device step `held` still owes: the zero step if
+// it halted there, then regular steps up to step_host.  Only zero_step and
+// advance leave work pending, so the system is mid-run.
+int resume(gdpd_ctx *ctx, Sys *s, long long held)
+{
+    const long long target = s->step_host;
+    int rc = GDPD_OK;
+    if (s->fix_path) {
+        s->fix_step = held;
+        for (long long n = held; n < target && !rc; ++n) rc = fix_iteration(ctx, s, n > 0, n > 0, true);
+        s->step_host = target;
+        return rc;
+    }
+    if (held == 0 && target >= 1) {
+        if ((rc = step_once(ctx, s->tab, kModeDrift))) return rc;
+        if ((rc = maybe_resort(ctx, s, s->tab, 1))) return rc;
+        held = 1;
+    }
+    if (target > held) {
+        const int gs = s->graph_steps > 0 ? s->graph_steps : 8;
+        if ((rc = advance_tables(ctx, s->tab, s->graph, (int)(target - held), gs, {s}))) return rc;
+    }
+    return GDPD_OK;
+}
+
+void drop_pending(gdpd_ctx *ctx, int id)
+{
+    ctx->pending.erase(std::remove(ctx->pending.begin(), ctx->pending.end(), id), ctx->pending.end());
+}
+
+// Read the halt flags of every system with enqueued steps; grow and resume
+// the held ones (each resumed system is checked again).
+int settle(gdpd_ctx *ctx)
+{
+    for (int round = 0; round < 16 && !ctx->pending.empty(); ++round) {
+        std::vector<int> ids;
+        for (int id : ctx->pending)
+            if (get_sys(ctx, id) && std::find(ids.begin(), ids.end(), id) == ids.end()) ids.push_back(id);
+        ctx->pending.clear();
+        std::vector<RepState> st;
+        int rc = read_states(ctx, ids, st);
+        if (rc) return rc;
+        for (size_t k = 0; k < ids.size(); ++k) {
+            if (!(st[k].flags & kFlagHalt)) continue;
+            Sys *s = ctx->sys[ids[k]];
+            if ((rc = regrow(ctx, s))) return rc;
+            if ((rc = resume(ctx, s, st[k].step))) return rc;
+            ctx->pending.push_back(ids[k]);
+        }
+    }
+    if (!ctx->pending.empty()) {
+        ctx->pending.clear();
+        return fail(ctx, GDPD_ERR_OVERFLOW, "brick staging kept overflowing after 16 capacity increases");
+    }
+    return GDPD_OK;
+}
+
+// After one synchronous step of s: true (and capacities grown) when staging
+// overflowed, so the caller repeats the step on the held state.
+int held_step(gdpd_ctx *ctx, Sys *s, bool &again)
+{
+    again = false;
+    RepState st;
+    int rc = read_state(ctx, s, &st);
+    if (rc) return rc;
+    if (!(st.flags & kFlagHalt)) return GDPD_OK;
+    if (s->regrows > 64) return fail(ctx, GDPD_ERR_OVERFLOW, "brick staging kept overflowing");
+    if ((rc = regrow(ctx, s))) return rc;
+    again = true;
+    return GDPD_OK;
+}
 } // namespace
 
 // =============================================================================
@@ -854,6 +1050,9 @@ int gdpd_create(gdpd_ctx **out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
         ctx->max_lds = (int)std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+        ctx->tick_ns = 1.0e6 / (double)khz;
     *out = ctx;
     return GDPD_OK;
 }
@@ -868,6 +1067,9 @@ int gdpd_destroy(gdpd_ctx *ctx)
     ctx->batch.release();
     ctx->batch_graph.release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    void *gb[] = {ctx->d_ids, ctx->d_states, ctx->d_ts, ctx->d_tsteps};
+    for (void *q : gb)
+        if (q) (void)hipFree(q);
     if (ctx->d_reps) (void)hipFree(ctx->d_reps);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -883,8 +1085,8 @@ int64_t gdpd_system_bytes(const gdpd_system_desc *d, int n_bonds)
     double vol = 1.0;
     for (int k = 0; k < 3; ++k) vol *= (d->hi[k] - d->lo[k]);
     const int64_t ncell = (int64_t)std::max(1.0, vol / std::max(d->global_cut * d->global_cut * d->global_cut, 1e-3));
-    // state x2 (96) + force/pe/bpe/vir/rho (96) + ints (36) + type
-    return N * 233 + ncell * 8 + (int64_t)n_bonds * 16 + (1 << 20);
+    // state x2 (96) + force/pe/bpe/ape/vir/rho (104) + ints and image word (44) + type
+    return N * 249 + ncell * 8 + (int64_t)n_bonds * 16 + (1 << 20);
 }
 
 int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
@@ -949,8 +1151,9 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     ALLOC(h.angle_start, (size_t)N + 1);
     ALLOC(h.vir, 6 * (size_t)N);
     ALLOC(h.rho, N);
-    ALLOC(h.iscratch, 3 * (size_t)N);
+    ALLOC(h.iscratch, 4 * (size_t)N);
     ALLOC(h.state, 1);
+    ALLOC(h.step_ts, kStepTsCap);
     ALLOC(h.racc, (size_t)kRecSlots * kAccPer * 2);
     ALLOC(h.rext, (size_t)kRecSlots * 2);
     ALLOC(h.rec_rows, (size_t)kRecSlots * kRecCap * kRecLen);
@@ -961,7 +1164,7 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     // empty bond CSR (bond_start[id] == 0 for all ids)
     ALLOC(h.bond_start, (size_t)N + 1);
 #undef ALLOC
-    h.scratch = nullptr;
+    HIPCHK(hipMemsetAsync(h.step_ts, 0, sizeof(unsigned long long) * kStepTsCap, ctx->stream));
     HIPCHK(hipMemsetAsync(h.bond_start, 0, sizeof(int) * ((size_t)N + 1), ctx->stream));
     HIPCHK(hipMemsetAsync(h.rec_n, 0, sizeof(int) * kRecSlots, ctx->stream));
     h.x0uw = nullptr;
@@ -986,6 +1189,7 @@ int gdpd_sys_destroy(gdpd_ctx *ctx, int id)
     Sys *s = get_sys(ctx, id);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", id);
     (void)hipStreamSynchronize(ctx->stream);
+    drop_pending(ctx, id);
     s->tab.release();
     s->graph.release();
     for (void *p : s->allocs) (void)hipFree(p);
@@ -1008,7 +1212,8 @@ int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const do
     RepDev &h = s->h;
     const int N = h.N;
     std::vector<double> X(3 * (size_t)N), V(3 * (size_t)N, 0.0);
-    std::vector<int> ids(N), img(N, pack_img(0, 0, 0)), soi(N, -1), perm(N);
+    std::vector<int> ids(N), soi(N, -1), perm(N);
+    std::vector<long long> img(N, pack_img(0, 0, 0));
     std::vector<unsigned char> ty(N);
     for (int k = 0; k < N; ++k) {
         if (id[k] < 1 || id[k] > N) return fail(ctx, GDPD_ERR_ARG, "atom id %d outside 1..%d", id[k], N);
@@ -1047,7 +1252,7 @@ int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const do
     HIPCHK(hipMemcpyAsync(h.v[1], V.data(), sizeof(double) * V.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(h.id, ids.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(h.type, ty.data(), N, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(h.img, img.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.img, img.data(), sizeof(long long) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(h.perm, perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(h.slot_of_id, soi.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1055,13 +1260,50 @@ int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const do
     s->cells_valid = false;
     s->midstep = false;
     s->nl_built = false;
+    s->step_host = 0;
+    drop_pending(ctx, sid);
     int rc = push_desc(ctx, sid);
     if (rc) return rc;
     if (s->prepared) {
-        // re-sort the fresh state into cell order
+        // re-sort the fresh state into cell order, then size the LDS
+        // capacities for it (the brick shape is kept while it fits)
+        launch_fill_i32(h.cell_count, 0, h.ncell, ctx->stream);
         if ((rc = rebin(ctx, s->tab))) return rc;
         launch_resort(ctx->d_reps, s->tab.beads, s->tab.n_beads, ctx->stream);
         if ((rc = check_launch(ctx))) return rc;
+        std::vector<int> cs((size_t)h.ncell + 1);
+        HIPCHK(hipMemcpyAsync(cs.data(), h.cell_start, sizeof(int) * cs.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const RepDev keep = h;
+        if ((rc = fit_capacities(ctx, s, cs, 0, 0))) {
+            // a state too dense for any brick: keep the brick geometry and
+            // require a fresh upload before the next computation
+            if (h.nbrick != keep.nbrick) {
+                (void)hipFree(h.thermo_part);
+                s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), (void *)h.thermo_part));
+                h.thermo_part = nullptr;
+                const int rc2 = dalloc(ctx, s, &h.thermo_part, (size_t)keep.nbrick * kThermoParts);
+                if (rc2) return rc2;
+            }
+            for (int d = 0; d < 3; ++d) {
+                h.bdim[d] = keep.bdim[d];
+                h.nbk[d] = keep.nbk[d];
+            }
+            h.nbrick = keep.nbrick;
+            h.smax = keep.smax;
+            h.omax = keep.omax;
+            s->uploaded = false;
+            (void)push_desc(ctx, sid);
+            (void)build_tables(ctx, {sid}, s->tab);
+            s->graph.release();
+            return rc;
+        }
+        if ((rc = apply_set_capacity(ctx, s))) return rc;
+        if ((rc = push_desc(ctx, sid))) return rc;
+        if ((rc = build_tables(ctx, {sid}, s->tab))) return rc;
+        s->graph.release();
+        ctx->batch_graph.release();
+        ctx->batch.release();
         s->cells_valid = true;
         s->steps_since_resort = 0;
     }
@@ -1075,9 +1317,10 @@ int gdpd_download(gdpd_ctx *ctx, int sid, double *pos, double *vel, int *image, 
     if (!s || !s->uploaded) return fail(ctx, GDPD_ERR_STATE, "system %d has no state", sid);
     RepDev &h = s->h;
     const int N = h.N;
-    RepState st;
-    int rc = read_state(ctx, s, &st);
+    int rc = settle(ctx);
     if (rc) return rc;
+    RepState st;
+    if ((rc = read_state(ctx, s, &st))) return rc;
     std::vector<int> ids(N);
     HIPCHK(hipMemcpy(ids.data(), h.id, sizeof(int) * N, hipMemcpyDeviceToHost));
     auto fetch3 = [&](const double *src, double *dst) -> int {
@@ -1091,8 +1334,8 @@ int gdpd_download(gdpd_ctx *ctx, int sid, double *pos, double *vel, int *image, 
     if (vel && (rc = fetch3(h.v[st.cur], vel))) return rc;
     if (force && (rc = fetch3(h.force, force))) return rc;
     if (image) {
-        std::vector<int> im(N);
-        HIPCHK(hipMemcpy(im.data(), h.img, sizeof(int) * N, hipMemcpyDeviceToHost));
+        std::vector<long long> im(N);
+        HIPCHK(hipMemcpy(im.data(), h.img, sizeof(long long) * N, hipMemcpyDeviceToHost));
         for (int k = 0; k < N; ++k) {
             int a, b, c;
             unpack_img(im[k], a, b, c);
@@ -1110,7 +1353,8 @@ int gdpd_download_pe_virial(gdpd_ctx *ctx, int sid, double *pe, double *virial, 
     if (!s || !s->uploaded) return fail(ctx, GDPD_ERR_STATE, "system %d has no state", sid);
     RepDev &h = s->h;
     const int N = h.N;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int src = settle(ctx);
+    if (src) return src;
     std::vector<int> ids(N);
     HIPCHK(hipMemcpy(ids.data(), h.id, sizeof(int) * N, hipMemcpyDeviceToHost));
     std::vector<double> tmp(6 * (size_t)N);
@@ -1372,20 +1616,26 @@ int gdpd_pair_compute(gdpd_ctx *ctx, int sid, uint32_t step, int log)
     Sys *s = get_sys(ctx, sid);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
     if (s->midstep) return fail(ctx, GDPD_ERR_STATE, "finish the fused run before using hooks");
-    int rc = ensure_cells(ctx, s);
+    int rc = settle(ctx);
     if (rc) return rc;
+    if ((rc = ensure_cells(ctx, s))) return rc;
     if ((rc = set_state_step(ctx, s, step, false))) return rc;
     const long long N = s->h.N;
-    if (log) {
-        launch_fill_f64(s->h.pe, 0.0, N, ctx->stream);
-        launch_fill_f64(s->h.vir, 0.0, 6 * N, ctx->stream);
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        if (log) {
+            launch_fill_f64(s->h.pe, 0.0, N, ctx->stream);
+            launch_fill_f64(s->h.vir, 0.0, 6 * N, ctx->stream);
+        }
+        if ((rc = force_launch(ctx, s->tab, kModeForceOnly | (log ? kModeLogForce : 0)))) return rc;
+        if (log) {
+            launch_fix_to_double(s->h.pe, N, s->h.precision, ctx->stream);
+            launch_fix_to_double(s->h.vir, 6 * N, s->h.precision, ctx->stream);
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        bool again = false;
+        if ((rc = held_step(ctx, s, again))) return rc;
+        if (!again) break;
     }
-    if ((rc = force_launch(ctx, s->tab, kModeForceOnly | (log ? kModeLogForce : 0)))) return rc;
-    if (log) {
-        launch_fix_to_double(s->h.pe, N, s->h.precision, ctx->stream);
-        launch_fix_to_double(s->h.vir, 6 * N, s->h.precision, ctx->stream);
-    }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     return check_flags(ctx, s);
 }
 
@@ -1479,10 +1729,13 @@ int gdpd_zero_step(gdpd_ctx *ctx, int sid)
     Sys *s = get_sys(ctx, sid);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
     int rc;
+    if ((rc = settle(ctx))) return rc;
     if (s->fix_path) {
         if ((rc = prepare(ctx, s))) return rc;
         if ((rc = fix_zero_step(ctx, s, true))) return rc;
         s->midstep = true;
+        s->step_host = 1;
+        ctx->pending.push_back(sid);
         return GDPD_OK;
     }
     if ((rc = ensure_cells(ctx, s))) return rc;
@@ -1490,6 +1743,8 @@ int gdpd_zero_step(gdpd_ctx *ctx, int sid)
     if ((rc = step_once(ctx, s->tab, kModeDrift))) return rc;
     if ((rc = maybe_resort(ctx, s, s->tab, 1))) return rc;
     s->midstep = true;
+    s->step_host = 1;
+    ctx->pending.push_back(sid);
     return GDPD_OK;
 }
 
@@ -1499,11 +1754,13 @@ int gdpd_advance(gdpd_ctx *ctx, int sid, int nsteps)
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
     if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "call gdpd_zero_step first");
     if (nsteps <= 0) return GDPD_OK;
+    int rc = settle(ctx);
+    if (rc) return rc;
+    s->step_host += nsteps;
+    ctx->pending.push_back(sid);
     if (s->fix_path) {
-        for (int k = 0; k < nsteps; ++k) {
-            int rc = fix_iteration(ctx, s, true, true, true);
-            if (rc) return rc;
-        }
+        for (int k = 0; k < nsteps; ++k)
+            if ((rc = fix_iteration(ctx, s, true, true, true))) return rc;
         return GDPD_OK;
     }
     const int gs = s->graph_steps > 0 ? s->graph_steps : 8;
@@ -1515,11 +1772,21 @@ int gdpd_finish(gdpd_ctx *ctx, int sid)
     Sys *s = get_sys(ctx, sid);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
     if (!s->midstep) return fail(ctx, GDPD_ERR_STATE, "call gdpd_zero_step first");
-    int rc = s->fix_path ? fix_iteration(ctx, s, true, true, false)
-                         : step_once(ctx, s->tab, kModeFinalKick | kModeStoreForce);
+    int rc = settle(ctx);
     if (rc) return rc;
+    const long long n = s->step_host;
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        if (s->fix_path) s->fix_step = n;
+        rc = s->fix_path ? fix_iteration(ctx, s, true, true, false)
+                         : step_once(ctx, s->tab, kModeFinalKick | kModeStoreForce);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        bool again = false;
+        if ((rc = held_step(ctx, s, again))) return rc;
+        if (!again) break;
+    }
     s->midstep = false;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    s->step_host = n + 1;
     return check_flags(ctx, s);
 }
 
@@ -1536,9 +1803,16 @@ int gdpd_run(gdpd_ctx *ctx, int sid, int nsteps)
         return check_flags(ctx, s);
     }
     if (nsteps == 0) {
+        if ((rc = settle(ctx))) return rc;
         if ((rc = set_state_step(ctx, s, 0, true))) return rc;
-        if ((rc = step_once(ctx, s->tab, kModeStoreForce))) return rc;
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (int attempt = 0; attempt < 16; ++attempt) {
+            if ((rc = step_once(ctx, s->tab, kModeStoreForce))) return rc;
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            bool again = false;
+            if ((rc = held_step(ctx, s, again))) return rc;
+            if (!again) break;
+        }
+        s->step_host = 1;
         return check_flags(ctx, s);
     }
     if ((rc = gdpd_zero_step(ctx, sid))) return rc;
@@ -1550,9 +1824,10 @@ int gdpd_thermo_rows(gdpd_ctx *ctx, int sid, double *out, int max_rows, int *n_r
 {
     Sys *s = get_sys(ctx, sid);
     if (!s || !n_rows) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
-    RepState st;
-    int rc = read_state(ctx, s, &st);
+    int rc = settle(ctx);
     if (rc) return rc;
+    RepState st;
+    if ((rc = read_state(ctx, s, &st))) return rc;
     const int cap = s->h.thermo_cap;
     const int have = std::min(st.log_rows, cap);
     const int first = st.log_rows - have;
@@ -1572,9 +1847,10 @@ int gdpd_get_step(gdpd_ctx *ctx, int sid, int64_t *step)
 {
     Sys *s = get_sys(ctx, sid);
     if (!s || !step) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
-    RepState st;
-    int rc = read_state(ctx, s, &st);
+    int rc = settle(ctx);
     if (rc) return rc;
+    RepState st;
+    if ((rc = read_state(ctx, s, &st))) return rc;
     *step = st.step;
     return GDPD_OK;
 }
@@ -1800,7 +2076,8 @@ int gdpd_records(gdpd_ctx *ctx, int sid, int index, double *out, int max_rows, i
 {
     Sys *s = get_sys(ctx, sid);
     if (!s || !n_rows || index < 0 || index >= kRecSlots) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int rc = settle(ctx);
+    if (rc) return rc;
     int nrec = 0;
     HIPCHK(hipMemcpy(&nrec, s->h.rec_n + index, sizeof(int), hipMemcpyDeviceToHost));
     const int have = std::min(nrec, kRecCap);
@@ -1832,6 +2109,8 @@ int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n, int nsteps)
         members.push_back(s);
     }
     int rc;
+    if (nsteps <= 0) return GDPD_OK;
+    if ((rc = settle(ctx))) return rc;
     if (ctx->batch.key != ids) {
         ctx->batch_graph.release();
         if ((rc = build_tables(ctx, ids, ctx->batch))) return rc;
@@ -1839,6 +2118,10 @@ int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n, int nsteps)
     int gs = 8;
     for (Sys *s : members)
         if (s->graph_steps > 0) gs = s->graph_steps;
+    for (size_t k = 0; k < members.size(); ++k) {
+        members[k]->step_host += nsteps;
+        ctx->pending.push_back(ids[k]);
+    }
     return advance_tables(ctx, ctx->batch, ctx->batch_graph, nsteps, gs, members);
 }
 
@@ -1856,6 +2139,26 @@ int gdpd_set_tuning(gdpd_ctx *ctx, int sid, int bx, int by, int bz, int resort_e
         s->graph_steps = graph_steps;
         s->graph.release();
     }
+    return GDPD_OK;
+}
+
+int gdpd_set_capacity(gdpd_ctx *ctx, int sid, int smax, int omax)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", sid);
+    if (smax < 0 || omax < 0 || smax > 65000 || omax > 65000)
+        return fail(ctx, GDPD_ERR_ARG, "capacities must be in 0..65000");
+    int rc = settle(ctx);
+    if (rc) return rc;
+    s->cap_s = smax;
+    s->cap_o = omax;
+    if (!s->prepared) return GDPD_OK;
+    if ((rc = apply_set_capacity(ctx, s))) return rc;
+    if ((rc = push_desc(ctx, sid))) return rc;
+    if ((rc = build_tables(ctx, {sid}, s->tab))) return rc;
+    s->graph.release();
+    ctx->batch_graph.release();
+    ctx->batch.release();
     return GDPD_OK;
 }
 
@@ -1911,6 +2214,28 @@ int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launc
 int gdpd_sync(gdpd_ctx *ctx)
 {
     if (!ctx) return GDPD_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return settle(ctx);
+}
+
+int gdpd_step_times(gdpd_ctx *ctx, const int *systems, int n, uint64_t *times, int64_t *steps, double *tick_ns)
+{
+    if (!ctx || n < 0 || (n > 0 && (!systems || !times || !steps))) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    int rc = settle(ctx);
+    if (rc) return rc;
+    if (tick_ns) *tick_ns = ctx->tick_ns;
+    if (n == 0) return GDPD_OK;
+    std::vector<int> ids(systems, systems + n);
+    for (int id : ids)
+        if (!get_sys(ctx, id)) return fail(ctx, GDPD_ERR_ARG, "no system %d", id);
+    std::vector<RepState> st;
+    if ((rc = read_states(ctx, ids, st))) return rc;  // sizes the gather buffers
+    HIPCHK(hipMemcpyAsync(ctx->d_ids, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_gather_step_ts(ctx->d_reps, ctx->d_ids, n, ctx->d_ts, ctx->d_tsteps, ctx->stream);
+    if ((rc = check_launch(ctx))) return rc;
+    static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "64-bit clock words");
+    HIPCHK(hipMemcpyAsync(times, ctx->d_ts, sizeof(uint64_t) * kStepTsCap * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(steps, ctx->d_tsteps, sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return GDPD_OK;
 }

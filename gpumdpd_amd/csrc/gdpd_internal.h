@@ -14,6 +14,7 @@ constexpr int kBlock = 256;        // threads per workgroup for every kernel
 constexpr int kThermoParts = 10;   // ke, pair pe, bond pe, p[6], angle pe
 constexpr int kThermoRowLen = 13;  // one device thermo record (GDPD_THERMO_LEN)
 constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kStepTsCap = 256;    // ring of step-completion times per replica (GDPD_STEP_TS_CAP)
 
 // group fixes and computes (GDPD_MAX_FIXES, GDPD_MAX_COMPUTES, GDPD_RECORD_*)
 constexpr int kMaxFixes = 8;
@@ -29,6 +30,13 @@ constexpr int kFlagListOverflow = 2;
 constexpr int kFlagNonFinite = 4;
 constexpr int kFlagNeighOverflow = 8;
 constexpr int kFlagWallOrder = 16;   // two_wall: the upper wall is not above the lower wall
+constexpr int kFlagImageOverflow = 32;  // a periodic image count would leave its 21-bit field
+constexpr int kFlagScatter = 64;     // a binned bead fell outside the bead range (internal error)
+// A replica whose brick staging overflowed is halted: every step kernel leaves
+// its state alone (no integration, no re-binning, no step count or buffer
+// flip), so the state stays at the last good step until the host has grown
+// the LDS capacities and cleared the flag.
+constexpr int kFlagHalt = kFlagStageOverflow;
 
 // step-kernel mode bits
 constexpr int kModeFinalKick = 1;  // v += dt/2 F/m before thermo (not at step 0)
@@ -93,7 +101,7 @@ struct RepDev {
     double *x[2], *v[2];       // SoA, 3N each: [X..., Y..., Z...]
     int *id;                   // 1-based ids
     unsigned char *type;       // 1-based types
-    int *img;                  // packed image counts, 10 bits per axis (offset 512)
+    long long *img;            // packed image counts, 21 bits per axis (offset 2^20)
     int *perm;                 // sorted position -> slot
     int *cell_start;           // ncell + 1
     int *cell_count;           // ncell, zero between steps
@@ -127,9 +135,9 @@ struct RepDev {
     int *rec_n;                // [kRecSlots]: records written
     double *x0uw;              // [3N] by id-1: msd reference (unwrapped) positions
     RepState *state;
+    unsigned long long *step_ts;  // [kStepTsCap]: device clock when step s completed, at s % kStepTsCap
     // scratch for re-sorting
-    double *scratch;           // 3N doubles
-    int *iscratch;             // 3N ints
+    int *iscratch;             // 4N ints: id, type, then the 64-bit image words
 };
 
 // One workgroup's work: replica index and the local chunk/brick index.
@@ -138,15 +146,28 @@ struct WorkItem {
     int local;
 };
 
-__host__ __device__ inline int pack_img(int ix, int iy, int iz)
+// Periodic image counts, 21 bits per axis with an offset of 2^20: +-1,048,575
+// box crossings per axis.  A count that would leave that range sets
+// kFlagImageOverflow instead of wrapping (the reference keeps the unwrapped
+// position in full precision, fix/fix_nve.cu:83-85).
+constexpr int kImgBias = 1 << 20;
+constexpr int kImgMax = kImgBias - 1;
+__host__ __device__ inline long long pack_img(int ix, int iy, int iz)
 {
-    return ((ix + 512) & 1023) | (((iy + 512) & 1023) << 10) | (((iz + 512) & 1023) << 20);
+    const long long m = (1ll << 21) - 1;
+    return ((long long)(ix + kImgBias) & m) | (((long long)(iy + kImgBias) & m) << 21) |
+           (((long long)(iz + kImgBias) & m) << 42);
 }
-__host__ __device__ inline void unpack_img(int p, int &ix, int &iy, int &iz)
+__host__ __device__ inline void unpack_img(long long p, int &ix, int &iy, int &iz)
 {
-    ix = (p & 1023) - 512;
-    iy = ((p >> 10) & 1023) - 512;
-    iz = ((p >> 20) & 1023) - 512;
+    const long long m = (1ll << 21) - 1;
+    ix = (int)(p & m) - kImgBias;
+    iy = (int)((p >> 21) & m) - kImgBias;
+    iz = (int)((p >> 42) & m) - kImgBias;
+}
+__host__ __device__ inline bool img_in_range(int ix, int iy, int iz)
+{
+    return ix >= -kImgMax && ix <= kImgMax && iy >= -kImgMax && iy <= kImgMax && iz >= -kImgMax && iz <= kImgMax;
 }
 
 } // namespace gdpd

@@ -35,6 +35,10 @@ void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int
 // per replica (rep_list: nreps replica indices)
 void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int flip, int advance,
                      int thermo, hipStream_t s, int bricks = 1);
+// scalar states and step-time rings of the listed replicas
+void launch_gather_state(const RepDev *reps, const int *rep_list, int nreps, RepState *out, hipStream_t s);
+void launch_gather_step_ts(const RepDev *reps, const int *rep_list, int nreps, unsigned long long *ts,
+                           long long *steps, hipStream_t s);
 // group fixes and computes: fmask / cmask select fixes / computes by index
 void launch_fix_clear(const RepDev *reps, const int *rep_list, int nreps, hipStream_t s);
 void launch_fix_initial(const RepDev *reps, const WorkItem *items, int nitems, unsigned fmask, hipStream_t s);

@@ -49,6 +49,10 @@ template <typename T> __device__ __forceinline__ __attribute__((address_space(1)
     return (__attribute__((address_space(1))) T *)p;
 }
 
+// A halted replica (kFlagHalt) keeps its state: the step kernels skip it whole.
+// The flag is per replica, so the test is uniform across a workgroup.
+__device__ __forceinline__ bool halted(const RepDev &R) { return (R.state->flags & kFlagHalt) != 0; }
+
 
 __device__ __forceinline__ int cell_axis(double x, double lo, double inv_cs, int nc)
 {
@@ -201,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_keys(const RepDev *__restrict__ 
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const double *X = R.x[R.state->cur];
     const int N = R.N;
     const int key = cell_of(R, X[s], X[N + s], X[2 * N + s]);
@@ -251,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const RepDev *__restrict__
     __shared__ int wbuf[kBlock / 64];
     const WorkItem it = items[blockIdx.x];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;  // the cell lists stay those of the held state
     const int base = it.local * 4 * kBlock + 4 * threadIdx.x;
     int v[4], loc = 0;
 #pragma unroll
@@ -279,11 +284,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const RepDev *__restrict__ r
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int key = gmem(R.key)[s];
     const int pos = ((unsigned)key < (unsigned)R.ncell) ? gmem(R.cell_start)[key] + gmem(R.rank)[s] : -1;
     if ((unsigned)pos < (unsigned)R.N) gmemw(R.perm)[pos] = s;
-    else atomicOr(&R.state->flags, kFlagStageOverflow);
+    else atomicOr(&R.state->flags, kFlagScatter);
 }
 
 // sort each cell's beads by id.  Used by the initial binning only, so a freshly
@@ -329,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_gather(const RepDev *__restri
     }
     R.iscratch[d] = R.id[s];
     R.iscratch[N + d] = R.type[s];
-    R.iscratch[2 * N + d] = R.img[s];
+    ((long long *)(R.iscratch + 2 * (size_t)N))[d] = R.img[s];
 }
 
 __global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restrict__ reps,
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restri
     const int id = R.iscratch[d];
     R.id[d] = id;
     R.type[d] = (unsigned char)R.iscratch[N + d];
-    R.img[d] = R.iscratch[2 * N + d];
+    R.img[d] = ((const long long *)(R.iscratch + 2 * (size_t)N))[d];
     R.perm[d] = d;
     if ((unsigned)(id - 1) < (unsigned)N) R.slot_of_id[id - 1] = d;
 }
@@ -1156,6 +1161,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;
     const int cur = R.state->cur;
     const BrickGeom g = brick_geom(R, it.local);
     const int O = stage_brick<real>(R, g, L, bp, cur, false, false);
@@ -1185,6 +1191,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;
     const int cur = R.state->cur;
     const long long step = R.state->step;
     const bool log = (mode & kModeLogForce) ||
@@ -1290,6 +1297,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
     __shared__ int hkey[kRankSlots], hcnt[kRankSlots], hbase[kRankSlots];
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;  // the force of this step is incomplete: hold the state
     const int cur = R.state->cur;
     const long long step = R.state->step;
     const bool log = (mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0;
@@ -1377,14 +1385,22 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             vxg = fma(h, fx, vxg);
             vyg = fma(h, fy, vyg);
             vzg = fma(h, fz, vzg);
-            int ix, iy, iz;
-            unpack_img(gmem(R.img)[s], ix, iy, iz);
-            const int ix0 = ix, iy0 = iy, iz0 = iz;
+            // box crossings of this drift; the image word is read and
+            // rewritten only for a bead that crossed
+            int jx = 0, jy = 0, jz = 0;
             bool b2 = false;
-            xg = wrap_axis(fma(R.dt, vxg, xg), R.lo[0], R.lo[0] + R.L[0], R.L[0], ix, b2);
-            yg = wrap_axis(fma(R.dt, vyg, yg), R.lo[1], R.lo[1] + R.L[1], R.L[1], iy, b2);
-            zg = wrap_axis(fma(R.dt, vzg, zg), R.lo[2], R.lo[2] + R.L[2], R.L[2], iz, b2);
-            if (ix != ix0 || iy != iy0 || iz != iz0) gmemw(R.img)[s] = pack_img(ix, iy, iz);
+            xg = wrap_axis(fma(R.dt, vxg, xg), R.lo[0], R.lo[0] + R.L[0], R.L[0], jx, b2);
+            yg = wrap_axis(fma(R.dt, vyg, yg), R.lo[1], R.lo[1] + R.L[1], R.L[1], jy, b2);
+            zg = wrap_axis(fma(R.dt, vzg, zg), R.lo[2], R.lo[2] + R.L[2], R.L[2], jz, b2);
+            if (jx | jy | jz) {
+                int ix, iy, iz;
+                unpack_img(gmem(R.img)[s], ix, iy, iz);
+                ix += jx;
+                iy += jy;
+                iz += jz;
+                if (img_in_range(ix, iy, iz)) gmemw(R.img)[s] = pack_img(ix, iy, iz);
+                else atomicOr(&R.state->flags, kFlagImageOverflow);
+            }
             if (b2 || !isfinite(vxg + vyg + vzg)) {
                 bad = true;
                 xg = R.lo[0];
@@ -1463,6 +1479,7 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
     __shared__ long long red[64];
     __shared__ long long itot[kThermoParts];
     const RepDev &R = reps[rep_list[blockIdx.x]];
+    if (halted(R)) return;  // no record, no step count, no buffer flip
     const long long step = R.state->step;
     if (thermo && R.log_every > 0 && (step % R.log_every) == 0) {
         // exact integer sums of the fixed-point partials
@@ -1510,9 +1527,29 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
         }
     }
     if (threadIdx.x == 0) {
+        if (advance) R.step_ts[step & (kStepTsCap - 1)] = wall_clock64();
         R.state->step = step + advance;
         R.state->cur ^= flip;
     }
+}
+
+// copies of the listed replicas' scalar states (one block per replica)
+__global__ void k_gather_state(const RepDev *__restrict__ reps, const int *__restrict__ rep_list,
+                               RepState *__restrict__ out)
+{
+    const RepDev &R = reps[rep_list[blockIdx.x]];
+    if (threadIdx.x == 0) out[blockIdx.x] = *R.state;
+}
+
+// the listed replicas' step-time rings and step counters (one block per replica)
+__global__ __launch_bounds__(kBlock) void k_gather_step_ts(const RepDev *__restrict__ reps,
+                                                           const int *__restrict__ rep_list,
+                                                           unsigned long long *__restrict__ ts,
+                                                           long long *__restrict__ steps)
+{
+    const RepDev &R = reps[rep_list[blockIdx.x]];
+    for (int k = threadIdx.x; k < kStepTsCap; k += kBlock) ts[(size_t)blockIdx.x * kStepTsCap + k] = R.step_ts[k];
+    if (threadIdx.x == 0) steps[blockIdx.x] = R.state->step;
 }
 
 // ---------------------------------------------------------------------------
@@ -1669,6 +1706,7 @@ __device__ __forceinline__ void spring_terms(const RepDev &R, const FixDev &F, i
 __global__ __launch_bounds__(kBlock) void k_fix_clear(const RepDev *__restrict__ reps, const int *__restrict__ rep_list)
 {
     const RepDev &R = reps[rep_list[blockIdx.x]];
+    if (halted(R)) return;
     for (int i = threadIdx.x; i < kRecSlots * kAccPer * 2; i += kBlock) R.racc[i] = 0;
     for (int i = threadIdx.x; i < kRecSlots; i += kBlock) {
         R.rext[2 * i] = 0ull;
@@ -1687,9 +1725,10 @@ __global__ __launch_bounds__(kBlock) void k_fix_initial(const RepDev *__restrict
     __shared__ long long lsum[kRecSlots * kAccPer * 2];
     __shared__ unsigned long long lext[kRecSlots * 2];
     FixBlockAcc A{lsum, lext};
-    fba_init(A);
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;
+    fba_init(A);
     const int s = it.local * kBlock + threadIdx.x;
     const bool live = s < R.N;
     const int N = R.N, cur = R.state->cur;
@@ -1742,7 +1781,8 @@ __global__ __launch_bounds__(kBlock) void k_fix_initial(const RepDev *__restrict
             X[d * N + s] = x[d];
             V[d * N + s] = v[d];
         }
-        R.img[s] = pack_img(im[0], im[1], im[2]);
+        if (img_in_range(im[0], im[1], im[2])) R.img[s] = pack_img(im[0], im[1], im[2]);
+        else atomicOr(&R.state->flags, kFlagImageOverflow);
         if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
     }
     bool any_ext = false;
@@ -1766,7 +1806,7 @@ __global__ __launch_bounds__(kBlock) void k_fix_two_wall(const RepDev *__restric
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N, cur = R.state->cur, type = R.type[s];
     double *X = R.x[cur], *V = R.v[cur];
     for (int k = 0; k < R.nfix; ++k) {
@@ -1802,9 +1842,10 @@ __global__ __launch_bounds__(kBlock) void k_fix_reduce(const RepDev *__restrict_
     __shared__ long long lsum[kRecSlots * kAccPer * 2];
     __shared__ unsigned long long lext[kRecSlots * 2];
     FixBlockAcc A{lsum, lext};
-    fba_init(A);
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;
+    fba_init(A);
     const int s = it.local * kBlock + threadIdx.x;
     const bool live = s < R.N;
     const int N = R.N, cur = R.state->cur;
@@ -1885,7 +1926,7 @@ __global__ __launch_bounds__(kBlock) void k_fix_post_force(const RepDev *__restr
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N, type = R.type[s];
     for (int k = 0; k < R.nfix; ++k) {
         if (!((fmask >> k) & 1u)) continue;
@@ -1907,7 +1948,7 @@ __global__ __launch_bounds__(kBlock) void k_fix_final(const RepDev *__restrict__
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N, type = R.type[s];
     double *V = R.v[R.state->cur];
     for (int k = 0; k < R.nfix; ++k) {
@@ -1930,7 +1971,7 @@ __global__ __launch_bounds__(kBlock) void k_fix_end(const RepDev *__restrict__ r
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N, type = R.type[s], cur = R.state->cur;
     double *X = R.x[cur], *V = R.v[cur];
     for (int k = 0; k < R.nfix; ++k) {
@@ -1951,7 +1992,8 @@ __global__ __launch_bounds__(kBlock) void k_fix_end(const RepDev *__restrict__ r
                 const double shift = F.p[d] - racc_value(R, slot, d) / F.gmass;
                 X[d * N + s] = wrap_axis(X[d * N + s] + shift, R.lo[d], R.lo[d] + R.L[d], R.L[d], im[d], bad);
             }
-            R.img[s] = pack_img(im[0], im[1], im[2]);
+            if (img_in_range(im[0], im[1], im[2])) R.img[s] = pack_img(im[0], im[1], im[2]);
+            else atomicOr(&R.state->flags, kFlagImageOverflow);
             if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
         }
     }
@@ -1962,7 +2004,7 @@ __global__ __launch_bounds__(64) void k_fix_records(const RepDev *__restrict__ r
                                                     unsigned fmask, unsigned cmask, long long step)
 {
     const RepDev &R = reps[rep_list[blockIdx.x]];
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0 || halted(R)) return;
     for (int k = 0; k < R.ncomp; ++k) {
         if (!((cmask >> k) & 1u)) continue;
         const CompDev &C = R.comp[k];
@@ -2006,7 +2048,7 @@ __global__ __launch_bounds__(kBlock) void k_nve(const RepDev *__restrict__ reps,
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N, cur = R.state->cur;
     double *X = R.x[cur], *V = R.v[cur];
     const double h = (0.5 * R.dt) * R.minv[R.type[s] - 1];
@@ -2024,7 +2066,8 @@ __global__ __launch_bounds__(kBlock) void k_nve(const RepDev *__restrict__ reps,
         }
     }
     if (initial) {
-        R.img[s] = pack_img(im[0], im[1], im[2]);
+        if (img_in_range(im[0], im[1], im[2])) R.img[s] = pack_img(im[0], im[1], im[2]);
+        else atomicOr(&R.state->flags, kFlagImageOverflow);
         if (bad) atomicOr(&R.state->flags, kFlagNonFinite);
     }
 }
@@ -2036,7 +2079,7 @@ __global__ __launch_bounds__(kBlock) void k_bond_hook(const RepDev *__restrict__
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N;
     const double *X = R.x[R.state->cur];
     const int id = R.id[s];
@@ -2082,7 +2125,7 @@ __global__ __launch_bounds__(kBlock) void k_angle_hook(const RepDev *__restrict_
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N) return;
+    if (s >= R.N || halted(R)) return;
     const int N = R.N;
     const double *X = R.x[R.state->cur];
     const int id = R.id[s];
@@ -2109,6 +2152,7 @@ __global__ __launch_bounds__(kBlock) void k_thermo_arrays(const RepDev *__restri
     __shared__ long long fred[64];
     const WorkItem it = items[blockIdx.x];
     const RepDev &R = reps[it.rep];
+    if (halted(R)) return;
     const int s = it.local * kBlock + threadIdx.x;
     double acc[kThermoParts];
 #pragma unroll
@@ -2368,6 +2412,17 @@ void launch_step_end(const RepDev *reps, const int *rep_list, int nreps, int fli
 {
     if (nreps)
         hipLaunchKernelGGL(k_step_end, dim3(nreps), dim3(kBlock), 0, s, reps, rep_list, flip, advance, thermo, bricks);
+}
+
+void launch_gather_state(const RepDev *reps, const int *rep_list, int nreps, RepState *out, hipStream_t s)
+{
+    if (nreps) hipLaunchKernelGGL(k_gather_state, dim3(nreps), dim3(64), 0, s, reps, rep_list, out);
+}
+
+void launch_gather_step_ts(const RepDev *reps, const int *rep_list, int nreps, unsigned long long *ts,
+                           long long *steps, hipStream_t s)
+{
+    if (nreps) hipLaunchKernelGGL(k_gather_step_ts, dim3(nreps), dim3(kBlock), 0, s, reps, rep_list, ts, steps);
 }
 void launch_fix_clear(const RepDev *reps, const int *rep_list, int nreps, hipStream_t s)
 {

@@ -12,12 +12,16 @@ Per-replica results do not depend on what else is in the batch: the kernels'
 per-bead sums are fixed-point and each replica keeps its own brick shape, so
 a replica gives the same bits alone and in any mix.
 
-Metrics (all wall-clock, host side, after a device sync per chunk):
-  replica-steps/s  total replica-steps / makespan (first arrival to last finish)
+Metrics:
+  replica-steps/s  total replica-steps / makespan (first arrival to last finish,
+                   host clock)
   bead-steps/s     total bead-steps / makespan
-  step latency     time between consecutive step completions of a replica; a
-                   chunk of k steps contributes k samples of (chunk time / k)
-  request latency  arrival to finish
+  step latency     time between consecutive step completions of a replica, one
+                   sample per step: the device writes a clock reading when each
+                   step's last kernel runs (gdpd_step_times), so the first step
+                   of a chunk includes the host turnaround and any wait to join
+                   the batch
+  request latency  arrival to finish (host clock)
 The plain baseline runs the same requests one at a time in arrival order,
 each alone on the GPU, with the same chunking.
 """
@@ -51,6 +55,21 @@ class Replica:
     t_last: float
     lat: list = field(default_factory=list)
     t_done: float = 0.0
+    seen: int = 0  # completed steps whose latency has been recorded (step 0 = the zero step)
+
+
+def _collect_latencies(ctx, reps):
+    """Append the per-step latencies (ms) of every step each replica completed
+    since the last call, from the device's step-completion clock."""
+    if not reps:
+        return
+    times, steps = ctx.step_times([a.sys for a in reps])
+    cap = times.shape[1]
+    for k, a in enumerate(reps):
+        done = int(steps[k]) - 1  # last completed step
+        for st in range(a.seen + 1, done + 1):
+            a.lat.append(times[k, st % cap] - times[k, (st - 1) % cap])
+        a.seen = max(a.seen, done)
 
 
 def make_requests(master_seed=5, n=512, rate=200.0, steps=2000, n_max=None, n_scale=1.0):
@@ -154,20 +173,19 @@ def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=32, precision=L.PREC
             batches += 1
         admit()
         ctx.sync()
-        tc = now()
         for a in active:
-            if k > 0:
-                a.lat.extend([(tc - a.t_last) / k] * k)
-                a.t_last = tc
-                a.remaining -= k
-        still = []
+            a.remaining -= k
+        _collect_latencies(ctx, active)
+        still, leaving = [], []
         for a in active:
             if a.remaining > 0:
                 still.append(a)
-                continue
-            a.sys.finish()
+            else:
+                a.sys.finish()
+                leaving.append(a)
+        _collect_latencies(ctx, leaving)
+        for a in leaving:
             td = now()
-            a.lat.append(td - a.t_last)
             a.t_done = td
             lat_all.extend(a.lat)
             req_lat.append(td - a.req.arrival)
@@ -175,7 +193,7 @@ def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=32, precision=L.PREC
             total_bead_steps += a.req.steps * a.req.n
             if a.req.k in keep:
                 d = a.sys.download()
-                kept[a.req.k] = (d["pos"], d["vel"], a.sys.thermo_rows())
+                kept[a.req.k] = (d["pos"], d["vel"], a.sys.thermo_rows(), d["force"])
             a.sys.destroy()
             used -= a.req.bytes
         active = still
@@ -183,9 +201,9 @@ def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=32, precision=L.PREC
     p50, p99 = _percentiles(lat_all)
     r50, r99 = _percentiles(req_lat)
     return {"replica_steps_per_s": total_steps / makespan, "bead_steps_per_s": total_bead_steps / makespan,
-            "makespan_s": makespan, "step_latency_p50_ms": p50 * 1e3, "step_latency_p99_ms": p99 * 1e3,
-            "request_latency_p50_s": r50, "request_latency_p99_s": r99, "batches": batches,
-            "requests": len(reqs), "replica_steps": total_steps, "kept": kept}
+            "makespan_s": makespan, "step_latency_p50_ms": p50, "step_latency_p99_ms": p99,
+            "step_latency_samples": len(lat_all), "request_latency_p50_s": r50, "request_latency_p99_s": r99,
+            "batches": batches, "requests": len(reqs), "replica_steps": total_steps, "kept": kept}
 
 
 def serve_plain(ctx, reqs, chunk=32, precision=L.PREC_F32, log_every=100, tuned=None, keep=None,
@@ -207,30 +225,28 @@ def serve_plain(ctx, reqs, chunk=32, precision=L.PREC_F32, log_every=100, tuned=
             time.sleep(r.arrival - now())
         s = _create(ctx, r, precision, log_every, tuned)
         s.zero_step()
-        t_last = now()
-        rem = r.steps - 1
-        while rem > 0:
-            k = min(chunk, rem)
+        a = Replica(r, s, r.steps - 1, now(), now())
+        while a.remaining > 0:
+            k = min(chunk, a.remaining)
             s.advance(k)
             ctx.sync()
-            tc = now()
-            lat_all.extend([(tc - t_last) / k] * k)
-            t_last = tc
-            rem -= k
+            a.remaining -= k
+            _collect_latencies(ctx, [a])
         s.finish()
+        _collect_latencies(ctx, [a])
+        lat_all.extend(a.lat)
         td = now()
-        lat_all.append(td - t_last)
         req_lat.append(td - r.arrival)
         total_steps += r.steps
         total_bead_steps += r.steps * r.n
         if r.k in keep:
             d = s.download()
-            kept[r.k] = (d["pos"], d["vel"], s.thermo_rows())
+            kept[r.k] = (d["pos"], d["vel"], s.thermo_rows(), d["force"])
         s.destroy()
     makespan = now() - (order[0].arrival if (order and realtime) else 0.0)
     p50, p99 = _percentiles(lat_all)
     r50, r99 = _percentiles(req_lat)
     return {"replica_steps_per_s": total_steps / makespan, "bead_steps_per_s": total_bead_steps / makespan,
-            "makespan_s": makespan, "step_latency_p50_ms": p50 * 1e3, "step_latency_p99_ms": p99 * 1e3,
-            "request_latency_p50_s": r50, "request_latency_p99_s": r99, "requests": len(reqs),
-            "replica_steps": total_steps, "kept": kept}
+            "makespan_s": makespan, "step_latency_p50_ms": p50, "step_latency_p99_ms": p99,
+            "step_latency_samples": len(lat_all), "request_latency_p50_s": r50, "request_latency_p99_s": r99,
+            "requests": len(reqs), "replica_steps": total_steps, "kept": kept}

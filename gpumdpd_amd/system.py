@@ -77,6 +77,18 @@ class Context:
                                            _ptr(r32, C.c_float), _ptr(r64, C.c_double)))
         return bits, r32, r64
 
+    def step_times(self, systems):
+        """(times (n, STEP_TS_CAP) in ms of the device clock, step counters (n,)):
+        the completion time of step s of system k sits at [k, s % STEP_TS_CAP]."""
+        ids = np.array([s.sid for s in systems], dtype=np.int32)
+        n = len(ids)
+        t = np.zeros((max(n, 1), L.STEP_TS_CAP), dtype=np.uint64)
+        st = np.zeros(max(n, 1), dtype=np.int64)
+        tick = C.c_double()
+        self.check(self.lib.gdpd_step_times(self.h, _ptr(ids, C.c_int), n, _ptr(t, C.c_uint64),
+                                            _ptr(st, C.c_int64), C.byref(tick)))
+        return t[:n].astype(np.float64) * (tick.value * 1e-6), st[:n]
+
     def kernel_timing(self, on):
         self.check(self.lib.gdpd_kernel_timing(self.h, 1 if on else 0))
 
@@ -293,6 +305,10 @@ class System:
     def set_tuning(self, brick=(0, 0, 0), resort_every=-1, graph_steps=-1):
         self.ctx.check(self.lib.gdpd_set_tuning(self.ctx.h, self.sid, int(brick[0]), int(brick[1]),
                                                 int(brick[2]), int(resort_every), int(graph_steps)))
+
+    def set_capacity(self, smax=0, omax=0):
+        """LDS capacities of the bricks (staged, owned beads; 0 = from the state)."""
+        self.ctx.check(self.lib.gdpd_set_capacity(self.ctx.h, self.sid, int(smax), int(omax)))
 
     def tuning(self):
         out = np.zeros(8, dtype=np.int32)

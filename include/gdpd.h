@@ -147,7 +147,16 @@ int gdpd_thermo(gdpd_ctx *ctx, int sys, double *out);
 int gdpd_run(gdpd_ctx *ctx, int sys, int nsteps);
 /* The same split in three for timing: zero_step leaves the system between
    steps (after step 1's drift); advance runs n whole steps; finish does the
-   last step's force and final half-kick. */
+   last step's force and final half-kick.
+   zero_step and advance only enqueue work.  Where the reference sizes its
+   neighbour list once from max_rho (force/neigh_list.cu:14-17), the bricks'
+   LDS capacities follow the state: a brick that outgrows them holds its system
+   at the last completed step (the rest of a batch goes on), and the next call
+   that waits for the device (advance, batch_advance, finish, download,
+   thermo_rows, get_step, sync, step_times) sizes the capacities again and runs
+   the held system's missing steps.  The result equals an uninterrupted run bit
+   for bit while the brick shape still fits in LDS (else the brick shrinks and
+   the run continues with FP32-rounding differences only). */
 int gdpd_zero_step(gdpd_ctx *ctx, int sys);
 int gdpd_advance(gdpd_ctx *ctx, int sys, int nsteps);
 int gdpd_finish(gdpd_ctx *ctx, int sys);
@@ -238,14 +247,31 @@ int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n_systems, int nst
    physical re-sorts of the bead arrays into cell order (0 = automatic). */
 int gdpd_set_tuning(gdpd_ctx *ctx, int sys, int bx, int by, int bz, int resort_every,
                     int graph_steps);
+/* out8: brick edges (3), LDS capacities smax and omax, re-sort interval, graph
+   length, brick count */
 int gdpd_get_tuning(gdpd_ctx *ctx, int sys, int *out8);
+/* LDS capacities of the bricks, staged and owned beads per brick (0 = sized
+   from the state with a margin).  The reference bounds its per-bead neighbour
+   storage by max_rho instead (force/neigh_list.cu:14-17).  A brick that
+   outgrows the capacities holds its system and the capacities are sized again
+   from the state (see gdpd_advance). */
+int gdpd_set_capacity(gdpd_ctx *ctx, int sys, int smax, int omax);
 /* Kernel timing: when on, every launch of the fused force kernel is bracketed by
    HIP events on the context stream; get returns the summed milliseconds and the
    launch count since the last reset. */
 int gdpd_kernel_timing(gdpd_ctx *ctx, int on);
 int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launches,
                            double *total_ms);
+/* waits for the context's stream, then settles held systems (see gdpd_advance) */
 int gdpd_sync(gdpd_ctx *ctx);
+/* Step-completion times of the listed systems, for per-step latency: times gets
+   GDPD_STEP_TS_CAP device-clock readings per system, the one of step s at
+   [k * GDPD_STEP_TS_CAP + s % GDPD_STEP_TS_CAP], written when the step's last
+   kernel ran; steps[k] gets the system's current step counter; tick_ns (may be
+   NULL) the clock period.  Read at least every GDPD_STEP_TS_CAP - 1 steps. */
+#define GDPD_STEP_TS_CAP 256
+int gdpd_step_times(gdpd_ctx *ctx, const int *systems, int n_systems, uint64_t *times, int64_t *steps,
+                    double *tick_ns);
 
 /* Evaluates the device Saru hash (the pair-force kernels' own inline function)
    for n host tuples: bits, the FP32-build uniform and the FP64-build uniform.

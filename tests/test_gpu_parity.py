@@ -70,16 +70,21 @@ def test_mdpd_density_and_force_c1(ctx, prec, tol):
     s.destroy()
 
 
-def test_bonded_melt_slice(ctx):
-    cfg = configs.config_c2(n_chains=1500, box=20.0)
-    print("case: C2 slice", flush=True)
+def test_bonded_melt_c2_full(ctx):
+    """The whole C2 diblock melt (374,992 beads, 351,555 bonds): pair + bond
+    force, per-bead pair PE and virial against the FP64 oracle."""
+    cfg = configs.config_c2()
+    print(f"case: C2 full, N={cfg['N']}", flush=True)
+    f_ref, pe_ref, _, vir_ref, _ = pyoracle.forces(cfg, step=5)
     for prec, tol in [(g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)]:
         s = g.system_from_config(ctx, cfg, precision=prec)
         s.pair_compute(step=5, log=True)
         s.bond_compute(step=5, log=True)
         f = s.download()["force"]
-        f_ref, _, _, _, _ = pyoracle.forces(cfg, step=5)
-        assert rel(f, f_ref) < tol
+        pe, vir, _ = s.download_pe_virial()
+        assert rel(f, f_ref) < tol, prec
+        assert rel(pe, pe_ref) < tol, prec
+        assert rel(vir, vir_ref) < tol, prec
         s.destroy()
 
 

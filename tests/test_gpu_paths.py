@@ -168,20 +168,22 @@ def test_bonds_added_after_graph_steps(ctx):
 
 
 def test_brick_capacity_overflow_is_reported_and_cleared_by_upload(ctx):
-    """LDS capacities are sized from the state at the first binning.  A later
-    state that packs far more beads into one brick than that makes the brick
-    kernel stop before staging (it checks the counts first) and set a sticky
-    flag, which the call reports as GDPD_ERR_OVERFLOW; a fresh upload clears it."""
+    """LDS capacities follow each uploaded state.  A state that packs more beads
+    into one cell than any brick can stage in LDS is refused at upload with
+    GDPD_ERR_OVERFLOW; the system keeps its brick and needs a fresh upload
+    before it computes again, after which it runs as if nothing had happened."""
     from gpumdpd_amd import _lib
     cfg = configs.scaled_dpd(6000, seed=31)
     s = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=10 ** 6)
     s.run(2)
     L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
     packed = np.asarray(cfg["lo"]) + 0.5 * L + np.random.default_rng(3).random((cfg["N"], 3)) * 1.5
-    s.upload(cfg["id"], cfg["type"], packed, cfg["vel"], cfg["masses"])
+    with pytest.raises(_lib.GdpdError) as e:
+        s.upload(cfg["id"], cfg["type"], packed, cfg["vel"], cfg["masses"])
+    assert e.value.code == 3 and "LDS staging" in str(e.value)
     with pytest.raises(_lib.GdpdError) as e:
         s.pair_compute(step=0, log=False)
-    assert e.value.code == 3 and "LDS capacity" in str(e.value)
+    assert e.value.code == 4
     s.upload(cfg["id"], cfg["type"], cfg["pos"], cfg["vel"], cfg["masses"])
     s.run(2)
     ref = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=10 ** 6)
