@@ -10,9 +10,11 @@ The served-mix half of the metric (512 Poisson replica requests, continuous
 batching, p50/p99 step latency, against the plain one-at-a-time path) is the
 "serving" object of the same line; --no-serve skips it.
 
-Prints one JSON line with the metric, the roofline of the dominant kernel (the
-brick pair-force kernel k_brick, timed with HIP events on the library stream it
-runs on) and a CPU baseline (the OpenMP oracle port on a bounded sample).
+The headline value times the shipped path: blocks of graph-captured steps
+replayed on the library's stream, kernel timing off.  A second pass of the same
+number of steps, eager, brackets every launch of the dominant kernel (the brick
+pair-force kernel k_brick) with HIP events on that stream; the roofline comes
+from it.  The CPU baseline is the OpenMP oracle port on bounded samples.
 
 Algorithmic bytes (compulsory HBM traffic, FP64 state, each array touched once):
   force pass (k_brick)   read x 24 + v 24 + id 4 + type 4, write f 24 = 80 B/bead
@@ -28,6 +30,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -40,14 +44,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds=12.0, n_target=1_000_000):
-    """OpenMP port of the reference force loop (oracle/) on a 1M-bead slice of the
-    same fluid, timed after warm-up; bead-steps/s."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_cpu_port(cfg, seconds, max_steps=200):
+    """Steps/s of the OpenMP port (oracle/) on cfg after 2 warm-up steps."""
     import ctypes as C
 
-    from gpumdpd_amd import configs
     from oracle import pyoracle
-    cfg = configs.scaled_dpd(n_target, seed=2024)
     s = pyoracle.make_orc_system(cfg, log_every=10 ** 9)
     L = pyoracle.lib()
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
@@ -60,12 +71,26 @@ def cpu_baseline(seconds=12.0, n_target=1_000_000):
         step += 1
         steps += 1
         el = time.perf_counter() - t0
-        if el > seconds or steps >= 200:
+        if el > seconds or steps >= max_steps:
             break
     L.orc_omp_destroy(h)
-    return {"value": cfg["N"] * steps / el, "unit": "bead-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{cfg['N']}-bead DPD fluid (rho=3, same force field), {steps} steps after 2 warm-up, "
-                      f"half neighbour list rebuilt on the skin criterion, {threads} OpenMP threads"}
+    return cfg["N"] * steps / el, steps, threads
+
+
+def cpu_baseline(seconds=12.0, n_target=1_000_000):
+    """OpenMP port of the reference force loop (oracle/): half neighbour list
+    with the skin check, the DPD force with per-thread force buffers and nve,
+    on the GPU box's host cores.  Headline: a 1M-bead slice of the 10M-bead
+    fluid; beside it the 24,000-bead C0 fluid."""
+    from gpumdpd_amd import configs
+    cfg = configs.scaled_dpd(n_target, seed=2024)
+    v, steps, threads = time_cpu_port(cfg, seconds)
+    c0 = configs.config_c0()
+    v0, steps0, _ = time_cpu_port(c0, min(5.0, seconds), max_steps=2000)
+    return {"value": v, "unit": "bead-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{cfg['N']}-bead DPD fluid (rho=3, same force field as the 10M-bead run), {steps} steps "
+                      f"after 2 warm-up, half neighbour list rebuilt on the skin criterion, {threads} OpenMP threads",
+            "c0": {"value": v0, "unit": "bead-steps/s", "n_beads": c0["N"], "steps": steps0}}
 
 
 def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
@@ -87,17 +112,21 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
     sample = reqs[::plain_every]
     p = serve.serve_plain(ctx, sample, chunk=chunk, tuned=tuned, realtime=False)
     p.pop("kept")
-    log(f"bench: plain sample of {len(sample)} in {p['makespan_s']:.1f}s")
+    log(f"bench: plain path on {len(sample)} requests in {p['makespan_s']:.1f}s")
     return {"metric": "served-mix replica-steps/s", "value": m["replica_steps_per_s"], "unit": "replica-steps/s",
             "step_latency_p50_ms": m["step_latency_p50_ms"], "step_latency_p99_ms": m["step_latency_p99_ms"],
+            "step_latency_samples": m["step_latency_samples"],
+            "step_latency_note": "per replica-step, from device step-completion clock readings",
             "request_latency_p50_s": m["request_latency_p50_s"], "request_latency_p99_s": m["request_latency_p99_s"],
             "bead_steps_per_s": m["bead_steps_per_s"], "makespan_s": m["makespan_s"], "requests": n_req,
             "steps_per_request": steps, "slots": slots, "chunk_steps": chunk,
             "tuned_bricks": {f"{k[0]}/{'small' if k[1] == 0 else 'large'}": v for k, v in tuned.items()},
             "plain": {"replica_steps_per_s": p["replica_steps_per_s"], "bead_steps_per_s": p["bead_steps_per_s"],
                       "step_latency_p50_ms": p["step_latency_p50_ms"], "step_latency_p99_ms": p["step_latency_p99_ms"],
-                      "sample": f"every {plain_every}th request ({len(sample)} of {n_req}), one at a time, "
-                                "back to back"},
+                      "makespan_s": p["makespan_s"],
+                      "sample": (f"all {n_req} requests" if plain_every == 1 else
+                                 f"every {plain_every}th request ({len(sample)} of {n_req})")
+                      + ", one at a time in arrival order, back to back"},
             "speedup_vs_plain": m["replica_steps_per_s"] / p["replica_steps_per_s"]}
 
 
@@ -128,7 +157,7 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=128)
     ap.add_argument("--serve-chunk", type=int, default=32)
-    ap.add_argument("--plain-every", type=int, default=8)
+    ap.add_argument("--plain-every", type=int, default=1)
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -160,16 +189,19 @@ def main():
     ctx.sync()
     log(f"bench: warm-up {args.warmup} steps done")
 
-    ctx.kernel_timing(True)
-    ctx.sync()
+    # the shipped path: graph-captured blocks replayed, no per-kernel events
     t0 = time.perf_counter()
     s.advance(args.steps)
     ctx.sync()
     wall = time.perf_counter() - t0
+    log(f"bench: timed {args.steps} steps in {wall:.3f}s")
+    # kernel pass: the same number of steps, eager, HIP events around each force launch
+    ctx.kernel_timing(True)
+    s.advance(args.steps)
+    ctx.sync()
     k_ms, k_n, _ = ctx.kernel_timing_get()
     ctx.kernel_timing(False)
     s.finish()
-    log(f"bench: timed {args.steps} steps in {wall:.3f}s")
 
     N = cfg["N"]
     value = N * args.steps / wall
@@ -194,14 +226,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if prec == g.PREC_F32 else "f64",
         "data": "synthetic",
-        "config": {"workload": workload, "n_beads": N, "box": list(map(float, cfg["hi"])), "rho": 3.0,
+        "config": {"workload": workload, "n_beads": N, "box": list(map(float, cfg["hi"])),
+                   "mean_rho": N / float(np.prod(np.asarray(cfg["hi"]) - np.asarray(cfg["lo"]))),
                    "precision": "fp32 pair force, fp64 accumulators and state" if prec == g.PREC_F32 else "fp64",
                    "brick": [tun["bx"], tun["by"], tun["bz"]], "resort_every": tun["resort_every"],
-                   "graph_steps": 0},
+                   "graph_steps": tun["graph_steps"], "timed_path": "graph-replayed fused steps"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(workload),
                      "kernel": "k_brick (pair force pass)", "kernel_ms": k_avg * 1e3,
                      "bytes_per_bead": per_kernel, "launches_timed": k_n,
+                     "kernel_timing": "HIP events around every k_brick launch of a second, eager pass of the "
+                                      "same number of steps on the library stream",
                      "step": {"bytes_per_bead_step": per, "achieved": step_gbs,
                               "frac": step_gbs / HBM_PEAK_GBS}},
     }
