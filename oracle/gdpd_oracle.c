@@ -676,6 +676,12 @@ static int build_list(orc_system *s, orc_work *w)
 int orc_run(orc_system *s, int nsteps, double *thermo_out, int max_rows, double *force_out,
             int *n_builds_out)
 {
+    return orc_run_observe(s, nsteps, thermo_out, max_rows, force_out, n_builds_out, 0, 0, 0);
+}
+
+int orc_run_observe(orc_system *s, int nsteps, double *thermo_out, int max_rows, double *force_out,
+                    int *n_builds_out, int observe_every, orc_observer observer, void *user)
+{
     int N = s->N;
     orc_work w;
     w.n_max = orc_n_max_neigh(s->global_cut, s->skin, s->max_rho);
@@ -718,6 +724,7 @@ int orc_run(orc_system *s, int nsteps, double *thermo_out, int max_rows, double 
                        w.bond_pe, thermo_out + 11 * rows);
             rows++;
         }
+        if (observer && observe_every > 0 && step % observe_every == 0) observer(step, s->pos, user);
     }
     if (force_out) memcpy(force_out, w.force, sizeof(double) * 3 * (size_t)N);
     if (n_builds_out) *n_builds_out = builds;

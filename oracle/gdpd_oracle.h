@@ -124,6 +124,11 @@ typedef struct {
    step's forces.  n_builds_out (may be NULL) counts list rebuilds. */
 int orc_run(orc_system *sys, int nsteps, double *thermo_out, int max_rows, double *force_out,
             int *n_builds_out);
+/* The same run calling observer(step, pos, user) after every observe_every-th
+   step's final integrate (pos: 3N wrapped positions, AoS). */
+typedef void (*orc_observer)(int step, const double *pos, void *user);
+int orc_run_observe(orc_system *sys, int nsteps, double *thermo_out, int max_rows, double *force_out,
+                    int *n_builds_out, int observe_every, orc_observer observer, void *user);
 
 /* ---- OpenMP CPU baseline: one full step (list check/rebuild, DPD force with
    per-thread force buffers, nve) -- same arithmetic as above.  Returns the number

@@ -38,6 +38,7 @@ class OrcSystem(C.Structure):
 
 
 _lib = None
+OBSERVER = C.CFUNCTYPE(None, C.c_int, C.POINTER(C.c_double), C.c_void_p)
 _PD, _PI, _PU = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_uint32)
 
 
@@ -73,6 +74,8 @@ def lib():
         L.orc_nve_final.argtypes = [C.c_int, C.c_double, _PI, _PD, _PD, _PD]
         L.orc_thermo.argtypes = [C.c_int, _PI, _PD, _PD, _PD, _PD, C.POINTER(Box), C.c_int, _PD, _PD]
         L.orc_run.argtypes = [C.POINTER(OrcSystem), C.c_int, _PD, C.c_int, _PD, _PI]
+        L.orc_run_observe.argtypes = [C.POINTER(OrcSystem), C.c_int, _PD, C.c_int, _PD, _PI, C.c_int,
+                                      OBSERVER, C.c_void_p]
         L.orc_omp_create.restype = C.c_void_p
         L.orc_omp_create.argtypes = [C.POINTER(OrcSystem), C.c_int]
         L.orc_omp_steps.argtypes = [C.c_void_p, C.c_uint32, C.c_int]
@@ -243,6 +246,22 @@ def run(cfg, nsteps, log_every=100, check_every=1):
         raise RuntimeError("oracle run: neighbour list overflow")
     k = s._keep
     return rows[:nr], {"pos": k["pos"], "vel": k["vel"], "uwpos": k["uwpos"], "force": force}, builds.value
+
+
+def run_observed(cfg, nsteps, every, fn, log_every=100, check_every=1):
+    """orc_run that calls fn(step, pos (N, 3) rows by id - 1) after every every-th step."""
+    s = make_orc_system(cfg, log_every, check_every)
+    N = cfg["N"]
+
+    def cb(step, pos, user):
+        fn(int(step), np.ctypeslib.as_array(pos, shape=(N * 3,)).reshape(N, 3).copy())
+
+    cfn = OBSERVER(cb)
+    rows = np.zeros((nsteps // log_every + 2, 11))
+    nr = lib().orc_run_observe(C.byref(s), int(nsteps), p(rows), len(rows), None, None, int(every), cfn, None)
+    if nr < 0:
+        raise RuntimeError("oracle run: neighbour list overflow")
+    return rows[:nr]
 
 
 # ---------------------------------------------------------------------------
