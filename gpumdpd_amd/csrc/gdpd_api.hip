@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,22 @@
 using namespace gdpd;
 
 namespace {
+
+// ---- allocation-free thermo: scratch in a static device array ----------------
+// gdpd_thermo_noalloc keeps its per-chunk partial sums and their total here
+// instead of in device memory allocated per call.  4 MiB: kThermoParts doubles
+// per 256-bead chunk, for systems of up to kNoAllocMaxChunks * 256 =
+// 13,421,568 beads, then the kThermoParts-double total at the end.  Every
+// call on a device shares that device's copy, which keeps the last call's
+// contents; calls that overlap on one device race.  Host code takes the
+// array's address per device with hipGetSymbolAddress at context creation
+// and passes that address to the kernels; it never uses the name itself.
+constexpr size_t kScratchDoubles = 512 * 1024;
+constexpr int kNoAllocMaxChunks = (int)(kScratchDoubles / kThermoParts) - 1;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {};  // per device: the address of its copy of scratch
+std::mutex g_scratch_mu;
 
 struct Tables {
     // device work tables for one set of systems
@@ -1053,6 +1070,16 @@ int gdpd_create(gdpd_ctx **out, int device)
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
         ctx->tick_ns = 1.0e6 / (double)khz;
+    // error messages fit the reserved capacity, so reporting one allocates nothing
+    ctx->err.reserve(600);
+    if (device < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_dev[device]) {
+            void *p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_dev[device] = (double *)p;
+            else (void)hipGetLastError();
+        }
+    }
     *out = ctx;
     return GDPD_OK;
 }
@@ -1709,6 +1736,45 @@ int gdpd_thermo(gdpd_ctx *ctx, int sid, double *out)
     double tot[kThermoParts] = {0};
     for (int b = 0; b < n; ++b)
         for (int k = 0; k < kThermoParts; ++k) tot[k] += hp[(size_t)b * kThermoParts + k];
+    const RepDev &h = s->h;
+    const double vol = h.L[0] * h.L[1] * h.L[2];
+    out[0] = (2.0 * tot[0]) / ((3.0 * h.N - 3.0) * 1.0);
+    out[1] = tot[1];
+    out[2] = tot[2];
+    for (int k = 0; k < 6; ++k) out[4 + k] = tot[3 + k] / vol;
+    out[3] = (out[4] + out[7] + out[9]) / 3.0;
+    out[10] = tot[0];
+    out[11] = -1.0;
+    out[12] = tot[9];
+    return GDPD_OK;
+}
+
+int gdpd_thermo_noalloc(gdpd_ctx *ctx, int sid, double *out)
+{
+    Sys *s = get_sys(ctx, sid);
+    if (!s || !out) return fail(ctx, GDPD_ERR_ARG, "bad arguments");
+    if (!s->prepared || !s->uploaded)
+        return fail(ctx, GDPD_ERR_STATE, "system %d: compute it once before the allocation-free thermo", sid);
+    const int n = s->tab.n_beads;
+    if (n > kNoAllocMaxChunks)
+        return fail(ctx, GDPD_ERR_ARG, "the allocation-free thermo takes at most %d beads (system %d has %d)",
+                    kNoAllocMaxChunks * kBlock, sid, s->h.N);
+    double *ptr = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_dev[ctx->device] : nullptr;
+    if (!ptr) return fail(ctx, GDPD_ERR_HIP, "no address for the device scratch array of device %d", ctx->device);
+    RepState st;
+    HIPCHK(hipMemcpyAsync(&st, s->h.state, sizeof(RepState), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (st.flags & kFlagHalt)
+        return fail(ctx, GDPD_ERR_STATE, "system %d is held after a staging overflow: call gdpd_sync first", sid);
+    // every chunk's partials are written before they are summed; nothing needs clearing
+    double *total = ptr + kScratchDoubles - kThermoParts;
+    launch_thermo_arrays(ctx->d_reps, s->tab.beads, n, ptr, ctx->stream);
+    launch_thermo_total(ptr, n, total, ctx->stream);
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    double tot[kThermoParts];
+    HIPCHK(hipMemcpyAsync(tot, total, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     const RepDev &h = s->h;
     const double vol = h.L[0] * h.L[1] * h.L[2];
     out[0] = (2.0 * tot[0]) / ((3.0 * h.N - 3.0) * 1.0);

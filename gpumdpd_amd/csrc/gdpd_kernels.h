@@ -56,6 +56,8 @@ void launch_bond_hook(const RepDev *reps, const WorkItem *items, int nitems, int
 void launch_angle_hook(const RepDev *reps, const WorkItem *items, int nitems, int log, hipStream_t s);
 void launch_thermo_arrays(const RepDev *reps, const WorkItem *items, int nitems, double *partials,
                           hipStream_t s);
+// one-thread, chunk-ordered total of nchunk partials into out[0..kThermoParts)
+void launch_thermo_total(const double *partials, int nchunk, double *out, hipStream_t s);
 // reference neighbour list (single replica)
 void launch_nl_bin(const RepDev *rep, int N, double bin, const int *nb, int *binof, int *count,
                    hipStream_t s);

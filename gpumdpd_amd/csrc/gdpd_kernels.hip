@@ -2140,6 +2140,22 @@ __global__ __launch_bounds__(kBlock) void k_angle_hook(const RepDev *__restrict_
     }
 }
 
+// Total of nchunk per-chunk thermo partials, summed by one thread in chunk
+// order so it rounds exactly as the host's sequential sum of the same
+// partials does.
+__global__ void k_thermo_total(const double *__restrict__ partials, int nchunk, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double tot[kThermoParts];
+#pragma unroll
+    for (int k = 0; k < kThermoParts; ++k) tot[k] = 0.0;
+    for (int b = 0; b < nchunk; ++b)
+#pragma unroll
+        for (int k = 0; k < kThermoParts; ++k) tot[k] += partials[(size_t)b * kThermoParts + k];
+#pragma unroll
+    for (int k = 0; k < kThermoParts; ++k) out[k] = tot[k];
+}
+
 // per-chunk thermo partials from the per-bead arrays: FP64 into partials, or
 // (partials == nullptr) fixed point into the replica's thermo_bead like the
 // fused integrate writes them
@@ -2476,6 +2492,10 @@ void launch_thermo_arrays(const RepDev *reps, const WorkItem *items, int nitems,
                           hipStream_t s)
 {
     if (nitems) hipLaunchKernelGGL(k_thermo_arrays, dim3(nitems), dim3(kBlock), 0, s, reps, items, partials);
+}
+void launch_thermo_total(const double *partials, int nchunk, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_thermo_total, dim3(1), dim3(64), 0, s, partials, nchunk, out);
 }
 void launch_nl_bin(const RepDev *rep, int N, double bin, const int *nb, int *binof, int *count,
                    hipStream_t s)

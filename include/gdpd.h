@@ -138,6 +138,14 @@ int gdpd_nve_final(gdpd_ctx *ctx, int sys);
 /* Thermo::process reductions (compute/thermo.cu:96-220) on the current state:
    out gets GDPD_THERMO_LEN doubles (step slot = -1). */
 int gdpd_thermo(gdpd_ctx *ctx, int sys, double *out);
+/* gdpd_thermo without any device or host allocation.  No counterpart in the
+   reference: for callers that cannot allocate (serving loops, harnesses that
+   forbid hipMalloc).  Its partial sums live in a fixed 4 MiB device array per
+   device, so it takes systems of at most 13,421,568 beads (larger ones get
+   GDPD_ERR_ARG) that have computed once (GDPD_ERR_STATE before that).  The
+   output equals gdpd_thermo's bit for bit.  Calls that overlap on one device
+   share the array and race. */
+int gdpd_thermo_noalloc(gdpd_ctx *ctx, int sys, double *out);
 
 /* ---- fused step path -------------------------------------------------------- */
 /* One `run N` (main/utils.cuh:166-314): zero_step at step 0, then steps 1..N, each
