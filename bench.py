@@ -102,7 +102,7 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
     from gpumdpd_amd import serve, tune
     t = time.perf_counter()
     tuned = tune.tune_classes(ctx, ("dpd", "mdpd"), g.PREC_F32)
-    log(f"bench: tuned bricks {tuned} in {time.perf_counter() - t:.1f}s")
+    log(f"bench: tuned {tuned} in {time.perf_counter() - t:.1f}s")
     reqs = serve.make_requests(5, n=n_req, steps=steps)
     serve.prepare_inputs(reqs)
     log(f"bench: serving {n_req} requests, {sum(r.n for r in reqs)} beads in total")
@@ -120,7 +120,7 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
             "request_latency_p50_s": m["request_latency_p50_s"], "request_latency_p99_s": m["request_latency_p99_s"],
             "bead_steps_per_s": m["bead_steps_per_s"], "makespan_s": m["makespan_s"], "requests": n_req,
             "steps_per_request": steps, "slots": slots, "chunk_steps": chunk,
-            "tuned_bricks": {f"{k[0]}/{'small' if k[1] == 0 else 'large'}": v for k, v in tuned.items()},
+            "tuned": {f"{k[0]}/{'small' if k[1] == 0 else 'large'}": v for k, v in tuned.items()},
             "plain": {"replica_steps_per_s": p["replica_steps_per_s"], "bead_steps_per_s": p["bead_steps_per_s"],
                       "step_latency_p50_ms": p["step_latency_p50_ms"], "step_latency_p99_ms": p["step_latency_p99_ms"],
                       "makespan_s": p["makespan_s"],

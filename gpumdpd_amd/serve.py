@@ -105,10 +105,12 @@ def _create(ctx, req, precision, log_every, tuned):
     import gpumdpd_amd as g
     from .tune import size_class
     s = g.system_from_config(ctx, req.cfg, precision=precision, log_every=log_every, upload=False)
-    # tuned: brick shape by (style, size class) or by style
-    b = tuned.get((req.style, size_class(req.n)), tuned.get(req.style))
-    if b:
-        s.set_tuning(brick=b)
+    # tuned: brick shape (and re-sort interval, graph length) by (style, size class) or by style
+    t = tuned.get((req.style, size_class(req.n)), tuned.get(req.style))
+    if isinstance(t, dict):
+        s.set_tuning(brick=t["brick"], resort_every=t.get("resort_every", -1), graph_steps=t.get("graph_steps", -1))
+    elif t:
+        s.set_tuning(brick=t)
     c = req.cfg
     s.upload(c["id"], c["type"], c["pos"], c["vel"], c["masses"])
     return s

@@ -1,16 +1,19 @@
-"""Brick-shape autotuner.
+"""Autotuner of the step's launch parameters, per pair style and replica size class.
 
-The force kernels stage one brick of bx*by*bz cells (plus a one-cell halo) per
-workgroup, so the brick shape is the kernel's tile.  The best tile depends on
-the bead density per cell (LDS capacity, occupancy) and on the pair style.
-The tuner times the force launches of a few candidate shapes on a sample system
-of the given class and keeps the fastest; results are cached per class for the
-life of the process.
+1. Brick shape.  The force kernels stage one brick of bx*by*bz cells (plus a
+   one-cell halo) per workgroup, so the brick shape is the kernel's tile.  The
+   best tile depends on the bead density per cell (LDS capacity, occupancy) and
+   on the pair style.  The tuner times the force launches of a few candidate
+   shapes on a sample system of the class and keeps the fastest.
+2. Schedule.  With that brick, it times whole graph-replayed steps for each
+   physical re-sort interval (bead arrays re-ordered into cell order every
+   `resort_every` steps) and each graph length (`graph_steps` steps captured
+   per hipGraph) of a small grid, and keeps the fastest pair.
+Results are cached per class for the life of the process.
 
 The other knobs of the reference's neighbour machinery do not exist in this
 design: cells are rebuilt every step (no Verlet skin, no rebuild interval), and
-the cell edge is the pair cutoff.  The physical re-sort interval and the number
-of steps per captured graph are tuned with the same timing loop.
+the cell edge is the pair cutoff.
 """
 import time
 
@@ -58,10 +61,58 @@ def size_class(n):
     return k
 
 
+RESORT_EVERY = (8, 16, 32)
+GRAPH_STEPS = (4, 8, 16)
+
+
 def tune_classes(ctx, styles, precision, verbose=False):
-    """Brick shape per (style, size class), for serve.serve's `tuned` table."""
-    return {(st, k): tune_brick(ctx, st, precision, n_sample=n, verbose=verbose)
-            for st in styles for k, (_, n) in enumerate(SIZE_CLASSES)}
+    """{(style, size class): {"brick", "resort_every", "graph_steps"}} for
+    serve.serve's `tuned` table."""
+    out = {}
+    for st in styles:
+        for k, (_, n) in enumerate(SIZE_CLASSES):
+            b = tune_brick(ctx, st, precision, n_sample=n, verbose=verbose)
+            r, gs = tune_schedule(ctx, st, precision, b, n_sample=n, verbose=verbose)
+            out[(st, k)] = {"brick": b, "resort_every": r, "graph_steps": gs}
+    return out
+
+
+def _time_steps(ctx, cfg, brick, resort_every, graph_steps, precision, steps):
+    import gpumdpd_amd as g
+    s = g.system_from_config(ctx, cfg, precision=precision, log_every=10 ** 6, upload=False)
+    try:
+        s.set_tuning(brick=brick, resort_every=resort_every, graph_steps=graph_steps)
+        s.upload(cfg["id"], cfg["type"], cfg["pos"], cfg["vel"], cfg["masses"])
+        s.zero_step()
+        s.advance(graph_steps + 1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        s.advance(steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        s.finish()
+        return wall / steps
+    finally:
+        s.destroy()
+
+
+def tune_schedule(ctx, style, precision, brick, n_sample=60000, steps=64, verbose=False):
+    """Fastest (resort_every, graph_steps) by whole-step time of graph-replayed
+    steps on an n_sample-bead replica with the given brick."""
+    key = ("schedule", style, precision, n_sample, tuple(brick))
+    if key in _CACHE:
+        return _CACHE[key]
+    cfg = configs.replica_system(style, n_sample, 12345)
+    best = None
+    for r in RESORT_EVERY:
+        for gs in GRAPH_STEPS:
+            t = _time_steps(ctx, cfg, brick, r, gs, precision, steps)
+            if verbose:
+                print(f"tune {style}: resort {r} graph {gs} step {t * 1e3:.4f} ms", flush=True)
+            if best is None or t < best[1]:
+                best = ((r, gs), t)
+    _CACHE[key] = best[0]
+    return best[0]
 
 
 def tune_brick(ctx, style, precision, n_sample=60000, steps=6, shapes=None, verbose=False):

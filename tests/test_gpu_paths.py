@@ -217,3 +217,17 @@ def test_batch_with_bonded_and_angle_members_equals_alone(ctx):
         assert np.array_equal(p, ref[0]) and np.array_equal(v, ref[1])
         assert np.array_equal(s.thermo_rows(), ref[2])
         s.destroy()
+
+
+def test_tuner_picks_brick_and_schedule(ctx):
+    """tune_classes' records carry the brick and the timed schedule, and a replica
+    created from such a record runs with them."""
+    from gpumdpd_amd import tune
+    b = tune.tune_brick(ctx, "dpd", g.PREC_F32, n_sample=4000, steps=3, shapes=[(3, 3, 3), (4, 4, 4)])
+    r, gs = tune.tune_schedule(ctx, "dpd", g.PREC_F32, b, n_sample=4000, steps=16)
+    assert r in tune.RESORT_EVERY and gs in tune.GRAPH_STEPS
+    s = g.system_from_config(ctx, configs.replica_system("dpd", 4000, 12345), precision=g.PREC_F32)
+    s.set_tuning(brick=b, resort_every=r, graph_steps=gs)
+    t = s.tuning()
+    assert (t["bx"], t["by"], t["bz"]) == tuple(b) and t["resort_every"] == r and t["graph_steps"] == gs
+    s.destroy()
