@@ -25,6 +25,10 @@ using namespace gdpd;
 
 namespace {
 
+// LDS of one gfx950 CU, and the share of each of four resident workgroups
+constexpr size_t kLdsPerCu = 160 * 1024;
+constexpr size_t kLdsFourPerCu = kLdsPerCu / 4;
+
 // ---- allocation-free thermo: scratch in a static device array ----------------
 // gdpd_thermo_noalloc keeps its per-chunk partial sums and their total here
 // instead of in device memory allocated per call.  4 MiB: kThermoParts doubles
@@ -275,13 +279,31 @@ int upload_items(gdpd_ctx *ctx, const std::vector<WorkItem> &v, WorkItem **d)
     return GDPD_OK;
 }
 
+// largest run-list size over the system's bricks: full bricks and the
+// shorter ones at the upper end of each axis
+int max_run_entries(const RepDev &h)
+{
+    int e[3][2], ne[3];
+    for (int d = 0; d < 3; ++d) {
+        ne[d] = 0;
+        e[d][ne[d]++] = std::max(1, std::min(h.bdim[d], h.nc[d]));
+        const int rem = h.nc[d] % std::max(1, h.bdim[d]);
+        if (rem > 0 && rem != e[d][0]) e[d][ne[d]++] = rem;
+    }
+    int best = 1;
+    for (int a = 0; a < ne[0]; ++a)
+        for (int b = 0; b < ne[1]; ++b)
+            for (int c = 0; c < ne[2]; ++c) best = std::max(best, brick_run_entries(e[0][a], e[1][b], e[2][c]));
+    return best;
+}
+
 BrickLaunch brick_params_of(const Sys *s)
 {
     BrickLaunch b{};
     b.rmax = (s->h.bdim[0] + 2) * (s->h.bdim[1] + 2) * (s->h.bdim[2] + 2);
     b.smax = s->h.smax;
     b.omax = s->h.omax;
-    b.ncol = s->h.bdim[0] * s->h.bdim[1];
+    b.nrun = max_run_entries(s->h);
     b.ntypes = s->h.ntypes;
     return b;
 }
@@ -291,7 +313,7 @@ void bl_max(BrickLaunch &a, const BrickLaunch &b)
     a.rmax = std::max(a.rmax, b.rmax);
     a.smax = std::max(a.smax, b.smax);
     a.omax = std::max(a.omax, b.omax);
-    a.ncol = std::max(a.ncol, b.ncol);
+    a.nrun = std::max(a.nrun, b.nrun);
     a.ntypes = std::max(a.ntypes, b.ntypes);
 }
 
@@ -472,9 +494,20 @@ int choose_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cell_start)
         maxreg = std::max(maxreg, cnt);
         maxown = std::max(maxown, own);
     }
-    // margins for density drift between re-checks
-    h.smax = std::min(65000, std::max(64, (int)std::ceil(maxreg * 1.12) + 32));
-    h.omax = std::min(65000, std::max(32, (int)std::ceil(maxown * 1.15) + 16));
+    // Margins for density drift: the largest of a few (up to 12% staged and
+    // 15% owned beads) whose LDS layout still lets four workgroups share a
+    // CU, the brick kernel's occupancy; a brick that outgrows them holds the
+    // run and the capacities are sized again (settle).
+    auto set = [&](double f) {
+        h.smax = std::min(65000, std::max(64, (int)std::ceil(maxreg * (1.0 + 0.12 * f)) + 16));
+        h.omax = std::min(65000, std::max(32, (int)std::ceil(maxown * (1.0 + 0.15 * f)) + 8));
+    };
+    const double fs[] = {1.0, 0.75, 0.5, 0.25, 0.0};
+    for (double f : fs) {
+        set(f);
+        if (brick_params_of(s).lds_bytes(h.precision, s->style) <= kLdsFourPerCu) return GDPD_OK;
+    }
+    set(1.0);
     (void)ctx;
     return GDPD_OK;
 }

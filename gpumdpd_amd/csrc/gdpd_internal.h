@@ -140,6 +140,38 @@ struct RepDev {
     int *iscratch;             // 4N ints: id, type, then the 64-bit image words
 };
 
+// Run-list entries an owned cell (rx, ry, rz), 1-based in a brick whose owned
+// extent is ex x ey x ez cells, may need in the brick kernel: the end of its own
+// column's run, one entry per candidate run that can hold beads (the four
+// following columns always; a preceding column's run when it is halo in x or y
+// or the cell touches a z face; the own column's bottom halo cell on the low z
+// face; the top halo cells of owned preceding columns in a brick one cell
+// thick in z), and a closing empty entry.  Depends on the geometry only.
+__host__ __device__ inline int cell_run_cap(int ex, int ey, int ez, int rx, int ry, int rz)
+{
+    const int RY = ey + 2;
+    const bool zlo = rz == 1, zhi = rz == ez, thin = ez == 1;
+    int n = 1 + 4 + 1 + (zlo ? 1 : 0);
+    for (int r = 0; r < 4; ++r) {
+        const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
+        const bool xyhalo = cx == 0 || cy == 0 || cy == RY - 1;
+        if (xyhalo || zlo || zhi) ++n;
+        if (thin && !xyhalo) ++n;
+    }
+    return n;
+}
+
+// Run-list entries of a whole brick: one shared empty entry, then every owned
+// cell's.
+inline int brick_run_entries(int ex, int ey, int ez)
+{
+    int n = 1;
+    for (int rx = 1; rx <= ex; ++rx)
+        for (int ry = 1; ry <= ey; ++ry)
+            for (int rz = 1; rz <= ez; ++rz) n += cell_run_cap(ex, ey, ez, rx, ry, rz);
+    return n;
+}
+
 // One workgroup's work: replica index and the local chunk/brick index.
 struct WorkItem {
     int rep;

@@ -8,7 +8,7 @@ struct BrickLaunch {
     int rmax;      // max region cells ((bx+2)(by+2)(bz+2)) over the launch
     int smax;      // max staged beads
     int omax;      // max owned beads of a brick
-    int ncol;      // max inner columns (bx*by)
+    int nrun;      // max run-list entries of a brick (brick_run_entries)
     int ntypes;    // max types
     int qw = 192;  // pair-queue entries per wave: a flush leaves < 64, one candidate step adds <= 128
     int perbead = 0; // per-bead PE/virial sums (hook path log steps)

@@ -374,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restri
 // force/pair_mdpd.cu:219-300, 403-517.
 
 struct BrickParams {
-    int rmax, smax, omax, ncol, ntypes, qw, perbead, style;
+    int rmax, smax, omax, nrun, ntypes, qw, perbead, style;
 };
 
 template <typename real> struct Vec4T;
@@ -414,12 +414,9 @@ __device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c)
     return r;
 }
 
-// Owned bead info: staged index k (16 bits) and its region cell (5, 5, 6 bits);
-// the host keeps brick edges within those fields.
-__device__ __forceinline__ int owned_info(int k, int rx, int ry, int rz)
-{
-    return k | (rx << 16) | (ry << 21) | (rz << 26);
-}
+// Owned bead info: staged index k (low 16 bits) and the base of its cell's run
+// list (high 16 bits).
+__device__ __forceinline__ int owned_info(int k, int rbase) { return k | (rbase << 16); }
 
 template <typename real> struct BrickLds {
     using real4 = typename Vec4T<real>::type;
@@ -429,7 +426,8 @@ template <typename real> struct BrickLds {
     int *sinfo;                // staged beads: owner index (owned) or slot (halo)
     real *rh;                  // staged density (mDPD force pass)
     int *c_start, *sigcnt, *wbuf, *oinfo;
-    int *cinfo, *gstart, *cellpre; // staging only (in the sums' space)
+    unsigned *runs;            // per owned cell: own-column end, candidate runs (js | je << 16), 0
+    int *cinfo, *gstart, *cellpre, *rbase; // staging only (in the sums' space)
     real4 *vown;               // owned beads: vxyz, w = slot bits
     long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
@@ -444,8 +442,8 @@ __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t
 // alone, and a pair with a halo bead reads the halo velocity from HBM.  This
 // keeps a 4x4x5-cell brick at rho = 3 under 40 KB, four workgroups per CU.
 struct LdsOffsets {
-    unsigned coef, red, pos, sinfo, rh, c_start, cinfo, gstart, cellpre, sigcnt, wbuf, oinfo, vown, facc, queue,
-        cellof, total;
+    unsigned coef, red, pos, sinfo, rh, c_start, cinfo, gstart, cellpre, rbase, sigcnt, wbuf, oinfo, runs, vown,
+        facc, queue, cellof, total;
 };
 
 template <typename real>
@@ -462,15 +460,17 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     GDPD_TAKE(rh, sizeof(real) * (p.style == 2 ? p.smax : 1))
     GDPD_TAKE(c_start, sizeof(int) * (p.rmax + 1))
     GDPD_TAKE(sigcnt, sizeof(int) * (kSigClasses + 1))
-    GDPD_TAKE(wbuf, sizeof(int) * (kBlock / 64))
+    GDPD_TAKE(wbuf, sizeof(int) * 2 * (kBlock / 64))
     GDPD_TAKE(oinfo, sizeof(int) * p.omax)
+    GDPD_TAKE(runs, sizeof(unsigned) * p.nrun)
     GDPD_TAKE(vown, sizeof(real) * 4 * p.omax)
-    // cinfo, gstart and cellpre live only while the brick is staged, before
-    // the per-bead sums are cleared, so they share the sums' space
+    // cinfo, gstart, cellpre and rbase live only while the brick is staged,
+    // before the per-bead sums are cleared, so they share the sums' space
     o.cinfo = (unsigned)off;
     o.gstart = (unsigned)(off + sizeof(int) * p.rmax);
     o.cellpre = (unsigned)(off + 2 * sizeof(int) * p.rmax);
-    GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 3 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 3 * 4 * (size_t)p.rmax)
+    o.rbase = (unsigned)(off + 3 * sizeof(int) * p.rmax);
+    GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 4 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 4 * 4 * (size_t)p.rmax)
     // the pair queues; the region cell of each staged index uses the same
     // space while the brick is staged, the block sums after the pair phase
     o.red = (unsigned)off;
@@ -500,6 +500,8 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
     L.cinfo = (int *)(smem + o.cinfo);
     L.gstart = (int *)(smem + o.gstart);
     L.cellpre = (int *)(smem + o.cellpre);
+    L.rbase = (int *)(smem + o.rbase);
+    L.runs = (unsigned *)(smem + o.runs);
     L.sigcnt = (int *)(smem + o.sigcnt);
     L.wbuf = (int *)(smem + o.wbuf);
     L.oinfo = (int *)(smem + o.oinfo);
@@ -569,15 +571,92 @@ __device__ __forceinline__ int real_to_id(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_longlong(v); }
 
 
-// Owned beads are numbered cell by cell, grouped by the cell's face class: the
-// brick faces the cell touches (x-low, y-low, y-high, z-low, z-high).  Cells of
-// one class have the same set of non-empty backward (halo-only) candidate
-// segments, so a wavefront of beads from one class skips the same segments
-// together; interior cells (class 0) come first.
-
-__device__ __forceinline__ int face_class(const BrickGeom &g, int rx, int ry, int rz)
+// The run list of owned cell (rx, ry, rz) at runs[0..]: the end of its own
+// column's candidate run, then every non-empty run of staged beads that a bead
+// of the cell owns the pair with, as js | je << 16, then 0.  A run is a z-run
+// of cells, contiguous in LDS, holding only such partners: later in LDS order,
+// or halo.  Per neighbour column (cx, cy) = (rx + r/3 - 1, ry + r%3 - 1):
+//   r 0..3  the columns that precede the cell's column: the whole z-run when
+//           the column is halo in x or y, else only its bottom halo cell
+//           (rz == 1) or its top one (rz == ext_z);
+//   r 4     the own column: its bottom halo cell (rz == 1 only); the part
+//           after the bead itself starts each bead's walk (the first entry);
+//   r 5..8  the four following columns: the whole z-run.
+// A brick one cell thick in z has both z-ends halo: an owned preceding column
+// then gives its bottom and its top halo cell, as two runs.  A pair of owned
+// beads in two columns is found from the bead whose column precedes, through
+// its following columns.  Returns the cell's candidate count (the own column
+// counted from the cell's first bead).  The entries written never exceed
+// cell_run_cap(ext, rx, ry, rz).
+__device__ __forceinline__ int cell_runs(const BrickGeom &g, const int *c_start, unsigned *runs, int rx, int ry,
+                                         int rz)
 {
-    return ((rx == 1) << 4) | ((ry == 1) << 3) | ((ry == g.ext[1]) << 2) | ((rz == 1) << 1) | (rz == g.ext[2]);
+    const int own = (rx * g.RY + ry) * g.RZ + rz;
+    const int dend = c_start[own + 2];
+    runs[0] = (unsigned)dend;
+    int n = 1, total = dend - c_start[own];
+    const bool zlo = rz == 1, zhi = rz == g.ext[2], thin = g.ext[2] == 1;
+    auto put = [&](int js, int je) {
+        if (je > js) {
+            runs[n++] = (unsigned)js | ((unsigned)je << 16);
+            total += je - js;
+        }
+    };
+    for (int r = 0; r < 9; ++r) {
+        const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
+        const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
+        const int A = c_start[cb], D = c_start[cb + 3];
+        if (r < 4) {
+            const bool xyhalo = cx == 0 || cy == 0 || cy == g.RY - 1;
+            if (xyhalo) {
+                put(A, D);
+            } else {
+                if (zlo) put(A, c_start[cb + 1]);
+                if (zhi && (thin || !zlo)) put(c_start[cb + 2], D);
+            }
+        } else if (r == 4) {
+            if (zlo) put(A, c_start[cb + 1]);
+        } else {
+            put(A, D);
+        }
+    }
+    runs[n] = 0u;
+    return total;
+}
+
+// exclusive scans of two values over the 256 threads (one pair per thread);
+// every thread must call it
+__device__ __forceinline__ void block_excl_scan2(int v0, int v1, int *wbuf, int &e0, int &e1, int &t0, int &t1)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = wave_incl_scan(v0), i1 = wave_incl_scan(v1);
+    if (lane == 63) {
+        wbuf[w] = i0;
+        wbuf[kBlock / 64 + w] = i1;
+    }
+    __syncthreads();
+    int o0 = 0, o1 = 0, s0 = 0, s1 = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+        const int a = wbuf[k], b = wbuf[kBlock / 64 + k];
+        o0 += (k < w) ? a : 0;
+        o1 += (k < w) ? b : 0;
+        s0 += a;
+        s1 += b;
+    }
+    __syncthreads();
+    e0 = o0 + i0 - v0;
+    e1 = o1 + i1 - v1;
+    t0 = s0;
+    t1 = s1;
+}
+
+// Owned cells are ordered by candidate count, largest first, in classes of
+// four candidates: the owned beads are numbered cell by cell in that order,
+// so the beads of one wavefront walk runs of about the same total length.
+__device__ __forceinline__ int cand_class(int total)
+{
+    return (kSigClasses - 1) - min(kSigClasses - 1, total >> 2);
 }
 
 // Barrier for LDS traffic only: it waits for this wave's LDS operations, not
@@ -637,23 +716,39 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         if (k < ncw) ((int *)L.coef)[k] = cw[u];
     }
     __syncthreads();
+    const int nown = g.ext[0] * g.ext[1] * g.ext[2];
     {
-        int v[4], loc = 0;
+        // region-cell starts, and the run-list base of every owned cell (after
+        // the shared empty entry runs[0]), in one pair of scans
+        int v[4], c[4], loc = 0, cloc = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int rc = 4 * t + k;
             v[k] = (rc < g.nreg) ? L.c_start[rc + 1] : 0;
             loc += v[k];
+            c[k] = 0;
+            if (rc < nown) {
+                int rx, ry, rz;
+                split3(rc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
+                c[k] = cell_run_cap(g.ext[0], g.ext[1], g.ext[2], rx + 1, ry + 1, rz + 1);
+            }
+            cloc += c[k];
         }
-        int total;
-        int run = block_excl_scan(loc, L.wbuf, total);
+        int run, crun, total, ctotal;
+        block_excl_scan2(loc, cloc, L.wbuf, run, crun, total, ctotal);
+        crun += 1;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int rc = 4 * t + k;
             if (rc < g.nreg) L.c_start[rc] = run;
+            if (rc < nown) L.rbase[rc] = crun;
             run += v[k];
+            crun += c[k];
         }
-        if (t == 0) L.c_start[g.nreg] = total;
+        if (t == 0) {
+            L.c_start[g.nreg] = total;
+            L.runs[0] = 0u;
+        }
         if (t <= kSigClasses) L.sigcnt[t] = 0;
     }
     __syncthreads();
@@ -715,15 +810,17 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     };
     lookup(t);
 
-    // owned-cell offsets: bead counts per face class (LDS atomics; the order
-    // of cells inside a class is immaterial), then a scan over the classes
-    const int nown = g.ext[0] * g.ext[1] * g.ext[2];
+    // each owned cell's run list; owned-cell offsets from bead counts per
+    // candidate class (LDS atomics; the order of cells inside a class is
+    // immaterial), kept in the low half of cellpre with the class above it,
+    // then a scan over the classes
     for (int oc = t; oc < nown; oc += kBlock) {
         int rx, ry, rz;
         split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
         ++rx, ++ry, ++rz;
         const int rc = (rx * g.RY + ry) * g.RZ + rz;
-        L.cellpre[oc] = atomicAdd(&L.sigcnt[face_class(g, rx, ry, rz)], L.c_start[rc + 1] - L.c_start[rc]);
+        const int cls = cand_class(cell_runs(g, L.c_start, L.runs + L.rbase[oc], rx, ry, rz));
+        L.cellpre[oc] = atomicAdd(&L.sigcnt[cls], L.c_start[rc + 1] - L.c_start[rc]) | (cls << 16);
     }
     lds_barrier();
     if (t < 64) {
@@ -735,9 +832,8 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     fetch();
     lds_barrier();
     for (int oc = t; oc < nown; oc += kBlock) {
-        int rx, ry, rz;
-        split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
-        L.cellpre[oc] += L.sigcnt[face_class(g, rx + 1, ry + 1, rz + 1)];
+        const int cp = L.cellpre[oc];
+        L.cellpre[oc] = (cp & 0xFFFF) + L.sigcnt[cp >> 16];
     }
     lds_barrier();
     const int O = L.sigcnt[kSigClasses];
@@ -768,7 +864,7 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
                 const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
                 const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
                 L.sinfo[k] = o;
-                L.oinfo[o] = owned_info(k, rx, ry, rz);
+                L.oinfo[o] = owned_info(k, L.rbase[oc]);
                 real4 q;
                 q.x = (real)vxs[u];
                 q.y = (real)vys[u];
@@ -985,16 +1081,19 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
 // Generation + evaluation.  The owned beads are split into 4 * nr equal chunks
 // of at most 64 (nr = ceil(O / 256) rounds), and wave w takes chunks w, w + 4,
 // ...: equal chunks keep the four waves' pair phases about equally long, so
-// few wait at the closing barrier.  Each lane scans its bead's 27 neighbour
-// cells as contiguous LDS segments that hold only the partners j the bead owns
-// the pair with: j after the bead in LDS order, or a halo bead.  A pair of
-// owned beads is thus found once and a pair with a halo bead once per brick.
-// In-range candidates are appended to a per-wave queue (ballot + mbcnt) that
-// is evaluated 64 pairs at a time.
+// few wait at the closing barrier.  Each lane walks its bead's candidates as
+// one sequence: the rest of its own column after the bead, then the runs of
+// its cell's run list (cell_runs), two candidates per step.  The walk does not
+// stop at run ends, so the wave's step count follows the longest total walk of
+// its lanes, not the longest run per column, and beads are numbered so that a
+// wave's beads have about the same total.  A pair of owned beads is found once
+// and a pair with a halo bead once per brick.  In-range candidates are
+// appended to a per-wave queue (ballot + mbcnt) that is evaluated 64 pairs at a
+// time.
 template <typename real, int STYLE, bool DENSITY>
-__device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g, const BrickLds<real> &L,
-                                            const BrickParams &bp, int O, int nt, real cut2, int cur,
-                                            uint32_t seed_step, bool log, long long (&acc)[kThermoParts])
+__device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickLds<real> &L, const BrickParams &bp, int O,
+                                            int nt, real cut2, int cur, uint32_t seed_step, bool log,
+                                            long long (&acc)[kThermoParts])
 {
     using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x, lane = t & 63;
@@ -1002,144 +1101,118 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickGeom &g,
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int qw = bp.qw;
     unsigned *q = L.queue + w * qw;
-    const bool thin = g.ext[2] == 1;
     int qn = 0;
     const int nwave = kBlock / 64;
     const int nchunk = nwave * max(1, (O + kBlock - 1) / kBlock);
     const int per = (O + nchunk - 1) / nchunk;  // <= 64; 0 when the brick owns no bead
+    // A step loads candidates j and j + 1 from pos[min(j, jmax)] on: a valid
+    // candidate is below the staged count S <= smax, and pos holds smax + 1
+    // entries, so both loads stay in bounds.
+    const unsigned jmax = (unsigned)bp.smax - 1u;
     for (int chunk = w; chunk < nchunk; chunk += nwave) {
         const int o = chunk * per + lane;
         const bool valid = lane < per && o < O;
-        int li = 0, rx = 1, ry = 1, rz = 1;
+        // walker: [wj, we) the current run, nxt the next one (js | je << 16;
+        // 0 ends the list), rr its index.  A lane without a bead starts on the
+        // shared empty entry runs[0] and stays there.
+        unsigned li = 0, wj = 0, we = 0, rr = 0, nxt = 0;
         real4 pi;
         pi.x = pi.y = pi.z = pi.w = (real)0;
         if (valid) {
             const int info = L.oinfo[o];
-            li = info & 0xFFFF;
-            rx = (info >> 16) & 31;
-            ry = (info >> 21) & 31;
-            rz = (info >> 26) & 63;
+            li = (unsigned)info & 0xFFFFu;
+            rr = (unsigned)info >> 16;
             pi = L.pos[li];
+            wj = li + 1u;
+            we = L.runs[rr];
+            rr += 1u;
+            nxt = L.runs[rr];
         }
-        // Ten candidate segments per bead, each a contiguous LDS range that
-        // holds only partners the bead owns the pair with:
-        //   s 0..3  z-runs of the four xy-neighbour columns that precede the
-        //           bead's column: whole run when the column is halo, else only
-        //           its bottom (rz == 1) or top (rz == RZ - 2) halo cell;
-        //   s 4     the bead's own column, bottom halo cell (rz == 1 only);
-        //   s 5     the bead's own column after the bead;
-        //   s 6..9  z-runs of the four following columns.
-        // A brick one cell thick in z has both z-ends halo: s 0..3 then take
-        // the bottom halo cell and s 10..13 the top one of an owned preceding
-        // column.  A pair of owned beads in two columns is found from the bead
-        // whose column precedes, through its following columns.
-        const int nseg = thin ? 14 : 10;
-        for (int s = 0; s < nseg; ++s) {
-            int js = 0, je = 0;
-            if (valid) {
-                const int r = s < 5 ? s : (s < 10 ? s - 1 : s - 10);
-                const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
-                const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
-                const int A = L.c_start[cb], D = L.c_start[cb + 3];
-                const bool zlo = (rz == 1), zhi = (rz + 1 == g.RZ - 1);
-                if (s < 4 || s >= 10) {
-                    const bool xyhalo = cx == 0 || cy == 0 || cy == g.RY - 1;
-                    if (s >= 10) {
-                        if (!xyhalo) {
-                            js = L.c_start[cb + 2];
-                            je = D;
-                        }
-                    } else if (xyhalo) {
-                        js = A;
-                        je = D;
-                    } else if (zlo) {
-                        js = A;
-                        je = L.c_start[cb + 1];
-                    } else if (zhi) {
-                        js = L.c_start[cb + 2];
-                        je = D;
-                    }
-                } else if (s == 4) {
-                    if (zlo) {
-                        js = A;
-                        je = L.c_start[cb + 1];
-                    }
-                } else if (s == 5) {
-                    js = li + 1;
-                    je = D;
-                } else {
-                    js = A;
-                    je = D;
-                }
+        // Move to the next run once the current one is used up (runs are not
+        // empty, so one move per step suffices), then read the entry after it.
+        // Branch-free, with the read every step: the LDS reads of a step then
+        // come in one fixed order, so the wait for this entry before the next
+        // move leaves the newer position loads in flight.
+        auto advance = [&]() {
+            const bool sw = wj >= we;
+            const unsigned nj = nxt & 0xFFFFu, ne = nxt >> 16;
+            wj = sw ? nj : wj;
+            we = sw ? ne : we;
+            rr += (sw && ne != 0u) ? 1u : 0u;
+            nxt = L.runs[rr];
+        };
+        // Candidates j, j + 1 against the queue: the in-range ones are
+        // appended (lanes past their walk are dropped by the validity masks).
+        // e1: the candidates' run end less one (signed: -1 for a finished
+        // walk), so candidate j is valid when j <= e1 and j + 1 when j < e1;
+        // the compares are made here, where the branches use them
+        auto test_pair = [&](const real4 &pj0, const real4 &pj1, unsigned j, int e1) {
+            const bool v0 = (int)j <= e1, v1 = (int)j < e1;
+            if (qn > qw - 128) {
+                // evaluate whole 64-pair rounds only; the remainder (< 64
+                // entries) moves to the front of the queue
+                const int ne = qn & ~63;
+                eval_queue<real, STYLE, DENSITY>(R, L, q, ne, nt, cur, seed_step, log, bp.perbead, acc);
+                const int rem = qn - ne;
+                if (lane < rem) q[lane] = q[ne + lane];
+                qn = rem;
             }
-            const int len = je - js;
-            if (__builtin_amdgcn_ballot_w64(len > 0) == 0) continue;
-            // queue entry of candidate k: li | (js + k) << 16
-            const unsigned ent0 = (unsigned)li | ((unsigned)js << 16);
-            // Candidates k, k + 1 against the queue: the in-range ones are
-            // appended (lanes past their segment are dropped by the k < len test).
-            auto test_pair = [&](const real4 &pj0, const real4 &pj1, int k) {
-                if (qn > qw - 128) {
-                    // evaluate whole 64-pair rounds only; the remainder (< 64
-                    // entries) moves to the front of the queue
-                    const int ne = qn & ~63;
-                    eval_queue<real, STYLE, DENSITY>(R, L, q, ne, nt, cur, seed_step, log, bp.perbead, acc);
-                    const int rem = qn - ne;
-                    if (lane < rem) q[lane] = q[ne + lane];
-                    qn = rem;
-                }
-                const real dx0 = pi.x - pj0.x, dy0 = pi.y - pj0.y, dz0 = pi.z - pj0.z;
-                const real dx1 = pi.x - pj1.x, dy1 = pi.y - pj1.y, dz1 = pi.z - pj1.z;
-                const real r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-                const real r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-                const bool ina = DENSITY ? (r20 < cut2) : (r20 <= cut2);
-                const bool inb = DENSITY ? (r21 < cut2) : (r21 <= cut2);
-                // Ballots of plain compares combine in SALU (a ballot of a
-                // combined bool is rematerialised per lane); the branches use
-                // the same compares as exec masks.
-                const unsigned long long ma = __builtin_amdgcn_ballot_w64(k < len) & __builtin_amdgcn_ballot_w64(ina);
-                const unsigned long long mb =
-                    __builtin_amdgcn_ballot_w64(k + 1 < len) & __builtin_amdgcn_ballot_w64(inb);
-                // append at the queue's uniform end, in lane order, the second
-                // candidate after the first
-                const unsigned ent = ent0 + ((unsigned)k << 16);
-                unsigned *qa = q + qn;
-                unsigned *qb = qa + __popcll(ma);
-                if (k < len && ina) qa[lanes_below(ma)] = ent;
-                if (k + 1 < len && inb) qb[lanes_below(mb)] = ent + 0x10000u;
-                qn += (int)(__popcll(ma) + __popcll(mb));
-            };
-            // Software-pipelined with two register sets, so no copies: the next
-            // pair of candidates is loaded before this pair is tested.  Candidate
-            // k + c sits at pj[k + c]; lanes past the staged range read its last
-            // entries instead (loads start at most at smax - 1; pos holds smax + 1
-            // entries).
-            const real4 *pj = L.pos + js;
-            const int lim = bp.smax - 1 - js;
-            real4 a0, a1, b0, b1;
+            const real dx0 = pi.x - pj0.x, dy0 = pi.y - pj0.y, dz0 = pi.z - pj0.z;
+            const real dx1 = pi.x - pj1.x, dy1 = pi.y - pj1.y, dz1 = pi.z - pj1.z;
+            const real r20 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+            const real r21 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+            const bool ina = DENSITY ? (r20 < cut2) : (r20 <= cut2);
+            const bool inb = DENSITY ? (r21 < cut2) : (r21 <= cut2);
+            // Ballots of plain compares combine in SALU (a ballot of a
+            // combined bool is rematerialised per lane); the branches use the
+            // same compares as exec masks.
+            const unsigned long long ma = __builtin_amdgcn_ballot_w64(v0) & __builtin_amdgcn_ballot_w64(ina);
+            const unsigned long long mb = __builtin_amdgcn_ballot_w64(v1) & __builtin_amdgcn_ballot_w64(inb);
+            // append at the queue's uniform end, in lane order, the second
+            // candidate after the first
+            const unsigned ent = li | (j << 16);
+            unsigned *qa = q + qn;
+            unsigned *qb = qa + __popcll(ma);
+            if (v0 && ina) qa[lanes_below(ma)] = ent;
+            if (v1 && inb) qb[lanes_below(mb)] = ent + 0x10000u;
+            qn += (int)(__popcll(ma) + __popcll(mb));
+        };
+        // Software-pipelined over two register sets, so no copies: the next
+        // step's walker move and loads come before this step's test.
+        advance();  // the own column may hold nothing after the bead
+        unsigned ja = wj;
+        int ea = (int)we - 1;
+        real4 pa0, pa1, pb0, pb1;
+        {
+            const unsigned lc = min(wj, jmax);
+            pa0 = L.pos[lc];
+            pa1 = L.pos[lc + 1u];
+        }
+        wj += 2u;
+        if (__builtin_amdgcn_ballot_w64((int)ja <= ea) == 0ull) continue;
+        for (;;) {
+            advance();
+            const unsigned jb = wj;
+            const int eb = (int)we - 1;
             {
-                const int lc = min(0, lim);
-                a0 = pj[lc];
-                a1 = pj[lc + 1];
+                const unsigned lc = min(wj, jmax);
+                pb0 = L.pos[lc];
+                pb1 = L.pos[lc + 1u];
             }
-            // the loop ends when no lane has a candidate left (a ballot of the
-            // next step's compare, in place of a wave maximum of len)
-            for (int k = 0;; k += 4) {
-                {
-                    const int lc = min(k + 2, lim);
-                    b0 = pj[lc];
-                    b1 = pj[lc + 1];
-                }
-                test_pair(a0, a1, k);
-                if (__builtin_amdgcn_ballot_w64(k + 2 < len) == 0) break;
-                {
-                    const int lc = min(k + 4, lim);
-                    a0 = pj[lc];
-                    a1 = pj[lc + 1];
-                }
-                test_pair(b0, b1, k + 2);
-                if (__builtin_amdgcn_ballot_w64(k + 4 < len) == 0) break;
+            wj += 2u;
+            test_pair(pa0, pa1, ja, ea);
+            if (__builtin_amdgcn_ballot_w64((int)jb <= eb) == 0ull) break;
+            advance();
+            ja = wj;
+            ea = (int)we - 1;
+            {
+                const unsigned lc = min(wj, jmax);
+                pa0 = L.pos[lc];
+                pa1 = L.pos[lc + 1u];
             }
+            wj += 2u;
+            test_pair(pb0, pb1, jb, eb);
+            if (__builtin_amdgcn_ballot_w64((int)ja <= ea) == 0ull) break;
         }
     }
     eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, cur, seed_step, log, bp.perbead, acc);
@@ -1172,7 +1245,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     long long acc[kThermoParts];
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
-    brick_pairs<real, 2, true>(R, g, L, bp, O, R.ntypes, (real)R.rdmax2, cur, 0u, false, acc);
+    brick_pairs<real, 2, true>(R, L, bp, O, R.ntypes, (real)R.rdmax2, cur, 0u, false, acc);
     for (int o = threadIdx.x; o < O; o += kBlock)
         R.rho[real_to_id(L.vown[o].w)] = (double)L.facc[o] * FixScale<real>::inv;
 }
@@ -1208,7 +1281,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     long long acc[kThermoParts];
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
-    brick_pairs<real, STYLE, false>(R, g, L, bp, O, R.ntypes, (real)R.rcmax2, cur, seed_step, log, acc);
+    brick_pairs<real, STYLE, false>(R, L, bp, O, R.ntypes, (real)R.rcmax2, cur, seed_step, log, acc);
     auto *F = gmemw(R.force);
     for (int o = t; o < O; o += kBlock) {
         const int slot = real_to_id(L.vown[o].w);
@@ -2352,7 +2425,7 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / 
 
 size_t BrickLaunch::lds_bytes(int precision, int style) const
 {
-    BrickParams p{rmax, smax, omax, ncol, ntypes, qw, perbead, style};
+    BrickParams p{rmax, smax, omax, nrun, ntypes, qw, perbead, style};
     return precision == 1 ? lds_offsets<double>(p).total : lds_offsets<float>(p).total;
 }
 
@@ -2388,7 +2461,7 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.ncol, bl.ntypes, bl.qw, 0, 1};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, 0, 1};
     const size_t lds = bl.lds_bytes(precision, 1);
     if (precision == 1)
         hipLaunchKernelGGL(k_density<double>, dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p);
@@ -2399,7 +2472,7 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
                   int precision, int style, int mode, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.ncol, bl.ntypes, bl.qw, bl.perbead, style};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, bl.perbead, style};
     const size_t lds = bl.lds_bytes(precision, style);
     if (precision == 1) {
         if (style == 2)
