@@ -251,8 +251,9 @@ int gdpd_records(gdpd_ctx *ctx, int sys, int index, double *out, int max_rows, i
 int gdpd_batch_advance(gdpd_ctx *ctx, const int *systems, int n_systems, int nsteps);
 
 /* ---- tuning and measurement (no reference counterpart) ---------------------- */
-/* brick edge in cells per axis (0 = automatic) and how many steps between
-   physical re-sorts of the bead arrays into cell order (0 = automatic). */
+/* brick edge in cells per axis (0 = automatic), how many steps between
+   physical re-sorts of the bead arrays into cell order (0 = automatic, 8) and
+   how many steps one captured graph holds (0 = automatic, 8). */
 int gdpd_set_tuning(gdpd_ctx *ctx, int sys, int bx, int by, int bz, int resort_every,
                     int graph_steps);
 /* out8: brick edges (3), LDS capacities smax and omax, re-sort interval, graph
