@@ -19,6 +19,10 @@ c1   8,000-bead mDPD liquid-vapour slab, 5,000 steps at dt = 0.01; the density
      profile along x (bins of 0.5) averaged over the last 2,000 steps, FP32
      mode against FP64 mode.
 c2   diblock melt (375k beads): FP32 vs FP64 force on the initial state.
+full each config's whole run on the fused path (C0 1,000 steps FP64, C1 5,000
+     FP32, C2 10,000 FP32, C3 500 FP32 and 500 FP64): the LDS capacities at
+     the start and at the end (they grow only if a brick outgrew them), and T
+     and P of the last thermo row.
 """
 import argparse
 import json
@@ -190,9 +194,33 @@ def section_c2(ctx):
             "fp32_vs_fp64_max_rel": max_rel(f32, f64)}
 
 
+def section_full(ctx):
+    out = {"section": "full", "runs": []}
+    runs = (("c0", configs.config_c0, 1000, g.PREC_F64, 100), ("c1", configs.config_c1, 5000, g.PREC_F32, 1000),
+            ("c2", configs.config_c2, 10000, g.PREC_F32, 1000), ("c3", configs.config_c3, 500, g.PREC_F32, 100),
+            ("c3", configs.config_c3, 500, g.PREC_F64, 100))
+    for name, make, nsteps, prec, every in runs:
+        cfg = make()
+        s = g.system_from_config(ctx, cfg, precision=prec, log_every=every)
+        t0 = s.tuning()
+        t = time.perf_counter()
+        s.run(nsteps)
+        wall = time.perf_counter() - t
+        t1 = s.tuning()
+        rows = s.thermo_rows()
+        s.destroy()
+        r = {"config": name, "n_beads": cfg["N"], "steps": nsteps, "precision": "fp64" if prec == g.PREC_F64 else "fp32",
+             "brick": [t1["bx"], t1["by"], t1["bz"]], "smax_omax_start": [t0["smax"], t0["omax"]],
+             "smax_omax_end": [t1["smax"], t1["omax"]], "T_end": float(rows[-1, 0]), "P_end": float(rows[-1, 3]),
+             "wall_s": round(wall, 2)}
+        out["runs"].append(r)
+        print(f"full: {r}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    for k in ("c3", "c3-oracle", "c0", "c0-oracle", "c1", "c2"):
+    for k in ("c3", "c3-oracle", "c0", "c0-oracle", "c1", "c2", "full"):
         ap.add_argument("--" + k, action="store_true")
     a = ap.parse_args()
     ctx = g.Context(0)
@@ -204,6 +232,8 @@ def main():
         print(json.dumps(section_c1(ctx)), flush=True)
     if a.c2:
         print(json.dumps(section_c2(ctx)), flush=True)
+    if a.full:
+        print(json.dumps(section_full(ctx)), flush=True)
     ctx.close()
 
 
