@@ -261,3 +261,70 @@ def test_tiny_and_ragged_boxes(ctx, n, box, style):
     assert len(rows) == len(ref_rows) == 5
     # one bead has no degrees of freedom left after the momentum term: T is inf in both
     assert np.allclose(rows[:, :11], ref_rows, rtol=1e-9, atol=1e-9, equal_nan=True)
+
+
+def _mixed(case):
+    """Multi-type systems whose pair cutoffs differ per type pair
+    (pair_coeff i j A gamma rc, force/pair_dpd.cu:118-160; mDPD adds B and rd,
+    pair_mdpd.cu:130-175): the search runs to the largest cutoff and each pair
+    is cut at its own rc (and weighted with its own rd)."""
+    if case == "dpd3":
+        box, rho, nt, style = (9.0, 8.5, 9.5), 3.0, 3, "dpd"
+        table = {(1, 1): (25.0, 4.5, 1.0), (1, 2): (40.0, 4.5, 0.8), (1, 3): (30.0, 3.0, 0.6),
+                 (2, 2): (25.0, 4.5, 0.9), (2, 3): (35.0, 2.0, 1.0), (3, 3): (20.0, 5.0, 0.7)}
+        masses = np.array([1.0, 1.5, 0.8])
+    elif case == "dpd_short":
+        box, rho, nt, style = (8.0, 8.0, 8.0), 3.0, 2, "dpd"
+        table = {(1, 1): (25.0, 4.5, 0.9), (1, 2): (30.0, 4.5, 0.75), (2, 2): (25.0, 4.5, 0.85)}
+        masses = np.array([1.0, 1.0])
+    else:
+        box, rho, nt, style = (7.0, 7.0, 7.0), 6.0, 2, "mdpd"
+        table = {(1, 1): (-40.0, 25.0, 4.5, 1.0, 0.75), (1, 2): (-30.0, 20.0, 4.5, 0.85, 0.65),
+                 (2, 2): (-35.0, 25.0, 3.0, 0.9, 0.7)}
+        masses = np.array([1.0, 1.2])
+    n = int(rho * np.prod(box))
+    cfg = configs.dpd_fluid(n, box, seed=900 + n)
+    typ = 1 + (configs.uniform(configs.substream(901, 2), n) * nt).astype(np.int32)
+    cfg.update(ntypes=nt, type=np.minimum(typ, nt).astype(np.int32), masses=masses, style=style, temp=1.0,
+               rc=1.0, saru_seed=42, skin=0.3, global_cut=1.0, name=f"mixed cutoffs {case}")
+    if style == "dpd":
+        cfg.update(dt=0.04, max_rho=6.0, coeff=configs.dpd_coeff(nt, table))
+    else:
+        cfg.update(dt=0.01, rd=0.75, max_rho=10.0, coeff=configs.mdpd_coeff(nt, table))
+    return cfg
+
+
+@pytest.mark.parametrize("case", ["dpd3", "dpd_short", "mdpd2"])
+def test_mixed_cutoffs_match_oracle(ctx, case):
+    cfg = _mixed(case)
+    print(f"case: mixed cutoffs {case}, N={cfg['N']}", flush=True)
+    f_ref, pe_ref, _, vir_ref, rho_ref = pyoracle.forces(cfg, step=4)
+    # the per-pair cutoffs matter: with every rc at 1.0 (rd 0.75) the force is far off
+    w = 3 if cfg["style"] == "dpd" else 5
+    cu = np.asarray(cfg["coeff"]).reshape(-1, w).copy()
+    cu[:, w - 1 if w == 3 else 3] = 1.0
+    if w == 5:
+        cu[:, 4] = 0.75
+    assert rel(pyoracle.forces(dict(cfg, coeff=cu.ravel()), step=4)[0], f_ref) > 0.1
+    for prec, tol in ((g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)):
+        s = g.system_from_config(ctx, cfg, precision=prec)
+        s.pair_compute(step=4, log=True)
+        f = s.download()["force"]
+        pe, vir, rho = s.download_pe_virial()
+        s.destroy()
+        assert rel(f, f_ref) < tol, prec
+        assert rel(pe, pe_ref) < tol and rel(vir, vir_ref) < tol, prec
+        if cfg["style"] == "mdpd":
+            assert rel(rho, rho_ref) < tol, prec
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F64, log_every=5)
+    s.run(20)
+    st = s.download()
+    rows = s.thermo_rows()
+    s.destroy()
+    ref_rows, ref_state, _ = pyoracle.run(cfg, 20, log_every=5)
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    d = st["pos"] - ref_state["pos"]
+    d -= L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-9
+    assert np.max(np.abs(st["vel"] - ref_state["vel"])) < 1e-8
+    assert np.allclose(rows[:, :11], ref_rows, rtol=1e-9, atol=1e-9)
