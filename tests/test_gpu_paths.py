@@ -31,9 +31,11 @@ def state(s):
     return d["pos"], d["vel"]
 
 
+@pytest.mark.parametrize("style", ["dpd", "mdpd"])
 @pytest.mark.parametrize("prec", [g.PREC_F32, g.PREC_F64])
-def test_hook_path_equals_fused_path(ctx, prec):
-    cfg = configs.scaled_dpd(5000, seed=21)
+def test_hook_path_equals_fused_path(ctx, prec, style):
+    """For mDPD the hook's pair_compute runs both passes (density, then force)."""
+    cfg = configs.scaled_dpd(5000, seed=21) if style == "dpd" else configs.config_c1()
     n = 12
     a = g.system_from_config(ctx, cfg, precision=prec, log_every=4)
     a.run(n)
@@ -66,18 +68,19 @@ def test_resort_interval_does_not_change_results(ctx):
         assert np.array_equal(o[2], out[0][2])
 
 
-def test_batch_equals_alone(ctx):
+@pytest.mark.parametrize("prec", [g.PREC_F32, g.PREC_F64])
+def test_batch_equals_alone(ctx, prec):
     reqs = [configs.replica_system("dpd", 3000, 5), configs.replica_system("mdpd", 2500, 6),
             configs.replica_system("dpd", 9000, 7)]
     alone = []
     for cfg in reqs:
-        s = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=5)
+        s = g.system_from_config(ctx, cfg, precision=prec, log_every=5)
         s.zero_step()
         s.advance(20)
         s.finish()
         alone.append(state(s) + (s.thermo_rows(),))
         s.destroy()
-    sys_ = [g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=5) for cfg in reqs]
+    sys_ = [g.system_from_config(ctx, cfg, precision=prec, log_every=5) for cfg in reqs]
     for s in sys_:
         s.zero_step()
     ctx.batch_advance(sys_, 20)
@@ -87,6 +90,19 @@ def test_batch_equals_alone(ctx):
         assert np.array_equal(p, ref[0]) and np.array_equal(v, ref[1])
         assert np.array_equal(s.thermo_rows(), ref[2])
         s.destroy()
+
+
+def test_batch_rejects_mixed_precision(ctx):
+    a = g.system_from_config(ctx, configs.replica_system("dpd", 2000, 9), precision=g.PREC_F32)
+    b = g.system_from_config(ctx, configs.replica_system("dpd", 2000, 10), precision=g.PREC_F64)
+    a.zero_step()
+    b.zero_step()
+    from gpumdpd_amd import _lib
+    with pytest.raises(_lib.GdpdError) as e:
+        ctx.batch_advance([a, b], 4)
+    assert "precision" in str(e.value)
+    a.destroy()
+    b.destroy()
 
 
 def test_cpp_plugin_shells_hook_equals_fused():
