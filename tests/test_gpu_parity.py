@@ -328,3 +328,21 @@ def test_mixed_cutoffs_match_oracle(ctx, case):
     assert np.max(np.abs(d)) < 1e-9
     assert np.max(np.abs(st["vel"] - ref_state["vel"])) < 1e-8
     assert np.allclose(rows[:, :11], ref_rows, rtol=1e-9, atol=1e-9)
+
+
+def test_coincident_beads(ctx):
+    """Two beads at one point add nothing to each other (force/pair_dpd.cu:
+    pairs with r < 1e-10 are skipped); a third bead 0.5 away feels both."""
+    cfg = _tiny(3, (4.0, 4.0, 4.0), seed=7)
+    cfg["pos"] = np.array([[1.5, 2.0, 2.0], [1.5, 2.0, 2.0], [2.0, 2.0, 2.0]])
+    cfg["vel"] = np.array([[0.3, -0.2, 0.1], [-0.1, 0.4, 0.0], [0.0, 0.0, -0.5]])
+    print("case: coincident beads", flush=True)
+    f_ref, pe_ref, _, _, _ = pyoracle.forces(cfg, step=3)
+    assert np.linalg.norm(f_ref[2]) > 1.0
+    for prec, tol in ((g.PREC_F32, F32_TOL), (g.PREC_F64, F64_TOL)):
+        s = g.system_from_config(ctx, cfg, precision=prec)
+        s.pair_compute(step=3, log=True)
+        f = s.download()["force"]
+        pe, _, _ = s.download_pe_virial()
+        s.destroy()
+        assert np.all(np.isfinite(f)) and rel(f, f_ref) < tol and rel(pe, pe_ref) < tol
