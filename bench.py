@@ -97,12 +97,15 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
     """Served mixed-replica stream (512 Poisson requests at 200/s, 70% DPD at
     rho=3 / 30% mDPD at rho=6, 2,000-200,000 beads log-uniform, 2,000 steps,
     FP32 force) with continuous batching, then the plain path (one replica at a
-    time, arrival order) on every plain_every-th request."""
+    time, arrival order) on every plain_every-th request.  Each path runs the
+    brick shapes the tuner picks for it: throughput shapes for the batch, and
+    for a replica alone the same list plus small bricks."""
     import gpumdpd_amd as g
     from gpumdpd_amd import serve, tune
     t = time.perf_counter()
     tuned = tune.tune_classes(ctx, ("dpd", "mdpd"), g.PREC_F32)
-    log(f"bench: tuned {tuned} in {time.perf_counter() - t:.1f}s")
+    tuned_alone = tune.tune_classes(ctx, ("dpd", "mdpd"), g.PREC_F32, alone=True)
+    log(f"bench: tuned {tuned} (alone: {tuned_alone}) in {time.perf_counter() - t:.1f}s")
     reqs = serve.make_requests(5, n=n_req, steps=steps)
     serve.prepare_inputs(reqs)
     log(f"bench: serving {n_req} requests, {sum(r.n for r in reqs)} beads in total")
@@ -110,7 +113,7 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
     m.pop("kept")
     log(f"bench: served in {m['makespan_s']:.1f}s")
     sample = reqs[::plain_every]
-    p = serve.serve_plain(ctx, sample, chunk=chunk, tuned=tuned, realtime=False)
+    p = serve.serve_plain(ctx, sample, chunk=chunk, tuned=tuned_alone, realtime=False)
     p.pop("kept")
     log(f"bench: plain path on {len(sample)} requests in {p['makespan_s']:.1f}s")
     return {"metric": "served-mix replica-steps/s", "value": m["replica_steps_per_s"], "unit": "replica-steps/s",
@@ -121,6 +124,7 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
             "bead_steps_per_s": m["bead_steps_per_s"], "makespan_s": m["makespan_s"], "requests": n_req,
             "steps_per_request": steps, "slots": slots, "chunk_steps": chunk,
             "tuned": {f"{k[0]}/{'small' if k[1] == 0 else 'large'}": v for k, v in tuned.items()},
+            "tuned_alone": {f"{k[0]}/{'small' if k[1] == 0 else 'large'}": v for k, v in tuned_alone.items()},
             "plain": {"replica_steps_per_s": p["replica_steps_per_s"], "bead_steps_per_s": p["bead_steps_per_s"],
                       "step_latency_p50_ms": p["step_latency_p50_ms"], "step_latency_p99_ms": p["step_latency_p99_ms"],
                       "makespan_s": p["makespan_s"],

@@ -11,6 +11,12 @@
    per hipGraph) of a small grid, and keeps the fastest pair.
 Results are cached per class for the life of the process.
 
+Two shape lists: the batch list (throughput shapes, for replicas stepped
+together) and, for a replica stepped alone, that list plus small bricks.  A
+small replica alone is latency-bound, so it runs faster with more, smaller
+bricks; in a batch the other replicas already fill the chip, and the same small
+bricks only add staging work (the served stream ran 11% slower with them).
+
 The other knobs of the reference's neighbour machinery do not exist in this
 design: cells are rebuilt every step (no Verlet skin, no rebuild interval), and
 the cell edge is the pair cutoff.
@@ -24,6 +30,9 @@ _CACHE = {}
 
 DPD_SHAPES = [(4, 4, 5), (4, 4, 4), (3, 4, 4), (4, 4, 6), (3, 4, 5)]
 MDPD_SHAPES = [(3, 3, 4), (3, 3, 3), (2, 3, 4), (3, 3, 5), (3, 4, 4)]
+# added for a replica stepped alone
+DPD_SHAPES_ALONE = [(3, 3, 3), (2, 3, 3), (2, 2, 3)]
+MDPD_SHAPES_ALONE = [(2, 2, 3), (2, 2, 2), (2, 3, 3)]
 
 
 def _time_force(ctx, cfg, brick, precision, steps):
@@ -65,13 +74,21 @@ RESORT_EVERY = (8, 16, 32)
 GRAPH_STEPS = (4, 8, 16)
 
 
-def tune_classes(ctx, styles, precision, verbose=False):
+def shapes_for(style, alone=False):
+    base = DPD_SHAPES if style == "dpd" else MDPD_SHAPES
+    if alone:
+        return base + (DPD_SHAPES_ALONE if style == "dpd" else MDPD_SHAPES_ALONE)
+    return base
+
+
+def tune_classes(ctx, styles, precision, alone=False, verbose=False):
     """{(style, size class): {"brick", "resort_every", "graph_steps"}} for
-    serve.serve's `tuned` table."""
+    serve.serve's `tuned` table (alone=True: for replicas stepped one at a
+    time, as serve.serve_plain does)."""
     out = {}
     for st in styles:
         for k, (_, n) in enumerate(SIZE_CLASSES):
-            b = tune_brick(ctx, st, precision, n_sample=n, verbose=verbose)
+            b = tune_brick(ctx, st, precision, n_sample=n, shapes=shapes_for(st, alone), verbose=verbose)
             r, gs = tune_schedule(ctx, st, precision, b, n_sample=n, verbose=verbose)
             out[(st, k)] = {"brick": b, "resort_every": r, "graph_steps": gs}
     return out
@@ -118,11 +135,11 @@ def tune_schedule(ctx, style, precision, brick, n_sample=60000, steps=64, verbos
 def tune_brick(ctx, style, precision, n_sample=60000, steps=6, shapes=None, verbose=False):
     """Fastest brick shape for replicas of `style` ("dpd" at rho=3, "mdpd" at
     rho=6) in `precision`, from force-kernel time on an n_sample-bead replica."""
-    key = (style, precision, n_sample)
+    shapes = shapes or shapes_for(style)
+    key = (style, precision, n_sample, tuple(map(tuple, shapes)))
     if key in _CACHE:
         return _CACHE[key]
     cfg = configs.replica_system(style, n_sample, 12345)
-    shapes = shapes or (DPD_SHAPES if style == "dpd" else MDPD_SHAPES)
     best = None
     for b in shapes:
         try:
