@@ -67,9 +67,10 @@ def _collect_latencies(ctx, reps):
     cap = times.shape[1]
     for k, a in enumerate(reps):
         done = int(steps[k]) - 1  # last completed step
-        for st in range(a.seen + 1, done + 1):
-            a.lat.append(times[k, st % cap] - times[k, (st - 1) % cap])
-        a.seen = max(a.seen, done)
+        if done > a.seen:
+            st = np.arange(a.seen + 1, done + 1)
+            a.lat.extend((times[k, st % cap] - times[k, (st - 1) % cap]).tolist())
+            a.seen = done
 
 
 def make_requests(master_seed=5, n=512, rate=200.0, steps=2000, n_max=None, n_scale=1.0):
