@@ -118,6 +118,7 @@ def serving_leg(ctx, n_req, steps, slots, chunk, plain_every):
     log(f"bench: plain path on {len(sample)} requests in {p['makespan_s']:.1f}s")
     return {"metric": "served-mix replica-steps/s", "value": m["replica_steps_per_s"], "unit": "replica-steps/s",
             "step_latency_p50_ms": m["step_latency_p50_ms"], "step_latency_p99_ms": m["step_latency_p99_ms"],
+            "step_latency_p999_ms": m["step_latency_p999_ms"],
             "step_latency_samples": m["step_latency_samples"],
             "step_latency_note": "per replica-step, from device step-completion clock readings",
             "request_latency_p50_s": m["request_latency_p50_s"], "request_latency_p99_s": m["request_latency_p99_s"],
@@ -160,7 +161,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=128)
-    ap.add_argument("--serve-chunk", type=int, default=32)
+    ap.add_argument("--serve-chunk", type=int, default=64)
     ap.add_argument("--plain-every", type=int, default=1)
     args = ap.parse_args()
     if args.copy_calibration:

@@ -124,7 +124,11 @@ def _percentiles(lat):
     return float(np.percentile(a, 50)), float(np.percentile(a, 99))
 
 
-def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=32, precision=L.PREC_F32, log_every=100,
+def _p999(lat):
+    return float(np.percentile(np.asarray(lat), 99.9)) if lat else None
+
+
+def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=64, precision=L.PREC_F32, log_every=100,
           tuned=None, keep=None, realtime=True):
     """Run the stream with continuous batching.  `keep`: request ids whose
     final state (pos, vel, thermo rows) is returned.  Returns a metrics dict."""
@@ -205,11 +209,12 @@ def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=32, precision=L.PREC
     r50, r99 = _percentiles(req_lat)
     return {"replica_steps_per_s": total_steps / makespan, "bead_steps_per_s": total_bead_steps / makespan,
             "makespan_s": makespan, "step_latency_p50_ms": p50, "step_latency_p99_ms": p99,
+            "step_latency_p999_ms": _p999(lat_all),
             "step_latency_samples": len(lat_all), "request_latency_p50_s": r50, "request_latency_p99_s": r99,
             "batches": batches, "requests": len(reqs), "replica_steps": total_steps, "kept": kept}
 
 
-def serve_plain(ctx, reqs, chunk=32, precision=L.PREC_F32, log_every=100, tuned=None, keep=None,
+def serve_plain(ctx, reqs, chunk=64, precision=L.PREC_F32, log_every=100, tuned=None, keep=None,
                 realtime=True):
     """Plain baseline: the same requests one at a time in arrival order."""
     tuned = tuned or {}
@@ -251,5 +256,6 @@ def serve_plain(ctx, reqs, chunk=32, precision=L.PREC_F32, log_every=100, tuned=
     r50, r99 = _percentiles(req_lat)
     return {"replica_steps_per_s": total_steps / makespan, "bead_steps_per_s": total_bead_steps / makespan,
             "makespan_s": makespan, "step_latency_p50_ms": p50, "step_latency_p99_ms": p99,
+            "step_latency_p999_ms": _p999(lat_all),
             "step_latency_samples": len(lat_all), "request_latency_p50_s": r50, "request_latency_p99_s": r99,
             "requests": len(reqs), "replica_steps": total_steps, "kept": kept}
