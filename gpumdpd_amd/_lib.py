@@ -93,6 +93,7 @@ PROTOTYPES = {
     "gdpd_kernel_timing": (_I, [_P, _I]),
     "gdpd_kernel_timing_get": (_I, [_P, _PD, C.POINTER(C.c_int64), _PD]),
     "gdpd_sync": (_I, [_P]),
+    "gdpd_admission": (_I, [_P, _I]),
     "gdpd_thermo_noalloc": (_I, [_P, _I, _PD]),
     "gdpd_set_capacity": (_I, [_P, _I, _I, _I]),
     "gdpd_step_times": (_I, [_P, _PI, _I, C.POINTER(C.c_uint64), C.POINTER(C.c_int64), _PD]),

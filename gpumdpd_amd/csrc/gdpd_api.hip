@@ -62,11 +62,16 @@ struct Tables {
     BrickLaunch bl_dpd{}, bl_mdpd{};
     int precision = 0;
     std::vector<int> key;
-    void release()
+    // defer: where to leave the buffers instead of freeing them now (hipFree
+    // waits for the whole device, including other streams' queued work)
+    void release(std::vector<void *> *defer = nullptr)
     {
         void *ps[] = {beads, cells, scan, bricks_dpd, bricks_mdpd, rep_chunk_off, rep_list, chunk_sums};
         for (void *p : ps)
-            if (p) (void)hipFree(p);
+            if (p) {
+                if (defer) defer->push_back(p);
+                else (void)hipFree(p);
+            }
         *this = Tables();
     }
 };
@@ -121,13 +126,23 @@ struct Sys {
     int type_count[kMaxTypes] = {0};
     bool fix_path = false;
     long long fix_step = 0;
+    bool admitted = false;  // created inside the current admission window
 };
 
 } // namespace
 
 struct gdpd_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the stream calls enqueue on: main, or side while admitting
+    hipStream_t main_stream = nullptr;
+    // admission window (gdpd_admission): new systems are set up on a second
+    // stream while the main stream keeps stepping the others
+    hipStream_t side_stream = nullptr;
+    hipEvent_t side_done = nullptr;
+    bool admitting = false, admit_refused = false;
+    // device buffers released inside an admission window, freed at the next
+    // point where the host has waited for the main stream anyway
+    std::vector<void *> graveyard;
     std::string err;
     std::vector<Sys *> sys;
     RepDev *d_reps = nullptr;
@@ -161,7 +176,12 @@ int fail(gdpd_ctx *c, int code, const char *fmt, ...)
     va_start(ap, fmt);
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
-    if (c) c->err = buf;
+    if (c) {
+        c->err = buf;
+        if (c->admit_refused)
+            c->err += " (inside an admission window only systems created in it can be used)";
+        c->admit_refused = false;
+    }
     return code;
 }
 
@@ -191,7 +211,21 @@ template <typename T> int dalloc(gdpd_ctx *ctx, Sys *s, T **p, size_t n)
 Sys *get_sys(gdpd_ctx *ctx, int id)
 {
     if (!ctx || id < 0 || id >= (int)ctx->sys.size()) return nullptr;
-    return ctx->sys[id];
+    Sys *s = ctx->sys[id];
+    if (s && ctx->admitting && !s->admitted) {
+        ctx->admit_refused = true;
+        return nullptr;
+    }
+    return s;
+}
+
+// frees the buffers left by admission windows; call only where the host has
+// just waited for the main stream
+void flush_graveyard(gdpd_ctx *ctx)
+{
+    if (ctx->admitting) return;
+    for (void *p : ctx->graveyard) (void)hipFree(p);
+    ctx->graveyard.clear();
 }
 
 int push_desc(gdpd_ctx *ctx, int id)
@@ -327,7 +361,7 @@ void bl_max(BrickLaunch &a, const BrickLaunch &b)
 
 int build_tables(gdpd_ctx *ctx, const std::vector<int> &ids, Tables &T)
 {
-    T.release();
+    T.release(ctx->admitting ? &ctx->graveyard : nullptr);
     std::vector<WorkItem> beads, cells, scan, bd, bm;
     std::vector<int> off(1, 0), reps;
     for (int id : ids) {
@@ -543,7 +577,8 @@ int fit_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cs_host, int s
         h.bdim[dmax]--;
         for (int d = 0; d < 3; ++d) h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
         h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
-        (void)hipFree(h.thermo_part);
+        if (ctx->admitting) ctx->graveyard.push_back(h.thermo_part);
+        else (void)hipFree(h.thermo_part);
         s->allocs.erase(std::find(s->allocs.begin(), s->allocs.end(), (void *)h.thermo_part));
         if ((rc = dalloc(ctx, s, &h.thermo_part, (size_t)h.nbrick * kThermoParts))) return rc;
         if ((rc = choose())) return rc;
@@ -1064,6 +1099,9 @@ void drop_pending(gdpd_ctx *ctx, int id)
 // the held ones (each resumed system is checked again).
 int settle(gdpd_ctx *ctx)
 {
+    // inside an admission window only new systems are reachable, and their
+    // steps are settled after it, on the main stream
+    if (ctx->admitting) return GDPD_OK;
     for (int round = 0; round < 16 && !ctx->pending.empty(); ++round) {
         std::vector<int> ids;
         for (int id : ctx->pending)
@@ -1072,6 +1110,7 @@ int settle(gdpd_ctx *ctx)
         std::vector<RepState> st;
         int rc = read_states(ctx, ids, st);
         if (rc) return rc;
+        flush_graveyard(ctx);
         for (size_t k = 0; k < ids.size(); ++k) {
             if (!(st[k].flags & kFlagHalt)) continue;
             Sys *s = ctx->sys[ids[k]];
@@ -1122,6 +1161,7 @@ int gdpd_create(gdpd_ctx **out, int device)
         delete ctx;
         return GDPD_ERR_HIP;
     }
+    ctx->main_stream = ctx->stream;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.sharedMemPerBlock > 0)
         ctx->max_lds = (int)std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
@@ -1146,9 +1186,12 @@ int gdpd_destroy(gdpd_ctx *ctx)
 {
     if (!ctx) return GDPD_OK;
     (void)hipSetDevice(ctx->device);
+    if (ctx->admitting) (void)gdpd_admission(ctx, 0);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side_stream) (void)hipStreamSynchronize(ctx->side_stream);
     for (size_t i = 0; i < ctx->sys.size(); ++i)
         if (ctx->sys[i]) gdpd_sys_destroy(ctx, (int)i);
+    flush_graveyard(ctx);
     ctx->batch.release();
     ctx->batch_graph.release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -1157,7 +1200,38 @@ int gdpd_destroy(gdpd_ctx *ctx)
         if (q) (void)hipFree(q);
     if (ctx->d_reps) (void)hipFree(ctx->d_reps);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side_done) (void)hipEventDestroy(ctx->side_done);
+    if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
     delete ctx;
+    return GDPD_OK;
+}
+
+int gdpd_admission(gdpd_ctx *ctx, int on)
+{
+    if (!ctx) return GDPD_ERR_ARG;
+    if (on) {
+        if (ctx->admitting) return GDPD_OK;
+        if (!ctx->side_stream) {
+            HIPCHK(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming));
+        }
+        if (ctx->timing) return fail(ctx, GDPD_ERR_STATE, "no admission window while kernel timing is on");
+        // New systems touch only their own arrays and their own slot of the
+        // descriptor table, which no queued main-stream work reads, so the side
+        // stream need not wait for the main one.  (A descriptor table that has
+        // to grow is freed by hipFree, which waits for the whole device.)
+        ctx->stream = ctx->side_stream;
+        ctx->admitting = true;
+        return GDPD_OK;
+    }
+    if (!ctx->admitting) return GDPD_OK;
+    // later main-stream work waits for everything the window enqueued
+    HIPCHK(hipEventRecord(ctx->side_done, ctx->side_stream));
+    HIPCHK(hipStreamWaitEvent(ctx->main_stream, ctx->side_done, 0));
+    ctx->stream = ctx->main_stream;
+    ctx->admitting = false;
+    for (Sys *s : ctx->sys)
+        if (s) s->admitted = false;
     return GDPD_OK;
 }
 
@@ -1187,7 +1261,9 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     if (d->precision != GDPD_PREC_F32 && d->precision != GDPD_PREC_F64)
         return fail(ctx, GDPD_ERR_ARG, "unknown precision %d", d->precision);
     HIPCHK(hipSetDevice(ctx->device));
+    const bool admitting = ctx->admitting;
     Sys *s = new Sys();
+    s->admitted = admitting;
     s->desc = *d;
     RepDev &h = s->h;
     const int N = d->n_beads;
@@ -1274,6 +1350,7 @@ int gdpd_sys_destroy(gdpd_ctx *ctx, int id)
     Sys *s = get_sys(ctx, id);
     if (!s) return fail(ctx, GDPD_ERR_ARG, "no system %d", id);
     (void)hipStreamSynchronize(ctx->stream);
+    flush_graveyard(ctx);
     drop_pending(ctx, id);
     s->tab.release();
     s->graph.release();
@@ -2339,6 +2416,7 @@ int gdpd_sync(gdpd_ctx *ctx)
 {
     if (!ctx) return GDPD_ERR_ARG;
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    flush_graveyard(ctx);
     return settle(ctx);
 }
 

@@ -149,17 +149,27 @@ def serve(ctx, reqs, max_slots=128, hbm_budget=200e9, chunk=64, precision=L.PREC
         return time.perf_counter() - t0
 
     def admit():
+        # new replicas are set up on the context's second stream while the
+        # batch's chunk runs on the main one
         nonlocal nxt, used
         t = now()
-        while (nxt < len(order) and (order[nxt].arrival <= t or not realtime) and len(active) + len(joining)
-               < max_slots and used + order[nxt].bytes <= hbm_budget):
-            r = order[nxt]
-            s = _create(ctx, r, precision, log_every, tuned)
-            s.zero_step()
-            used += r.bytes
-            joining.append(Replica(r, s, r.steps - 1, t, t))
-            nxt += 1
-            t = now()
+        opened = False
+        try:
+            while (nxt < len(order) and (order[nxt].arrival <= t or not realtime) and len(active) + len(joining)
+                   < max_slots and used + order[nxt].bytes <= hbm_budget):
+                if not opened:
+                    ctx.admission(True)
+                    opened = True
+                r = order[nxt]
+                s = _create(ctx, r, precision, log_every, tuned)
+                s.zero_step()
+                used += r.bytes
+                joining.append(Replica(r, s, r.steps - 1, t, t))
+                nxt += 1
+                t = now()
+        finally:
+            if opened:
+                ctx.admission(False)
 
     joining = []
     while nxt < len(order) or active or joining:

@@ -62,6 +62,12 @@ class Context:
     def sync(self):
         self.check(self.lib.gdpd_sync(self.h))
 
+    def admission(self, on):
+        """Open (True) or close (False) an admission window: systems created and
+        zero-stepped inside it are set up on a second stream, beside the work
+        already queued for the others."""
+        self.check(self.lib.gdpd_admission(self.h, 1 if on else 0))
+
     def batch_advance(self, systems, nsteps):
         ids = np.array([s.sid for s in systems], dtype=np.int32)
         self.check(self.lib.gdpd_batch_advance(self.h, _ptr(ids, C.c_int), len(ids), int(nsteps)))

@@ -273,6 +273,16 @@ int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launc
                            double *total_ms);
 /* waits for the context's stream, then settles held systems (see gdpd_advance) */
 int gdpd_sync(gdpd_ctx *ctx);
+/* Admission window, for serving loops that admit new systems while others run.
+   No counterpart in the reference (it runs one system).  Between
+   gdpd_admission(ctx, 1) and gdpd_admission(ctx, 0) the calls that create and
+   set up new systems (gdpd_sys_create, gdpd_set_*, gdpd_upload, gdpd_zero_step)
+   run on a second stream of the context and do not wait for work already
+   queued on its main stream; only systems created inside the window can be
+   used there.  Closing the window makes all later main-stream work wait for
+   the window's work, without blocking the host.  Results are the same as
+   outside a window. */
+int gdpd_admission(gdpd_ctx *ctx, int on);
 /* Step-completion times of the listed systems, for per-step latency: times gets
    GDPD_STEP_TS_CAP device-clock readings per system, the one of step s at
    [k * GDPD_STEP_TS_CAP + s % GDPD_STEP_TS_CAP], written when the step's last

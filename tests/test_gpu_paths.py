@@ -247,3 +247,40 @@ def test_tuner_picks_brick_and_schedule(ctx):
     t = s.tuning()
     assert (t["bx"], t["by"], t["bz"]) == tuple(b) and t["resort_every"] == r and t["graph_steps"] == gs
     s.destroy()
+
+
+def test_admission_window(ctx):
+    """Systems created and zero-stepped inside an admission window run on the
+    context's second stream beside queued main-stream work; they, and the
+    systems already running, end with the same bits as without the window.
+    Inside the window the running systems are out of reach."""
+    from gpumdpd_amd import _lib
+    ca, cb = configs.replica_system("dpd", 30000, 13), configs.replica_system("mdpd", 6000, 14)
+    ref = []
+    for cfg, n in ((ca, 39 + 19), (cb, 19)):
+        s = g.system_from_config(ctx, cfg, precision=g.PREC_F32, log_every=5)
+        s.zero_step()
+        s.advance(n)
+        s.finish()
+        ref.append(state(s) + (s.thermo_rows(),))
+        s.destroy()
+    print("case: admission window", flush=True)
+    a = g.system_from_config(ctx, ca, precision=g.PREC_F32, log_every=5)
+    a.zero_step()
+    a.advance(39)  # queued on the main stream
+    ctx.admission(True)
+    try:
+        b = g.system_from_config(ctx, cb, precision=g.PREC_F32, log_every=5)
+        b.zero_step()
+        with pytest.raises(_lib.GdpdError) as e:
+            a.advance(1)
+        assert "admission window" in str(e.value)
+    finally:
+        ctx.admission(False)
+    ctx.batch_advance([a, b], 19)
+    for s, r in ((a, ref[0]), (b, ref[1])):
+        s.finish()
+        p, v = state(s)
+        assert np.array_equal(p, r[0]) and np.array_equal(v, r[1])
+        assert np.array_equal(s.thermo_rows(), r[2])
+        s.destroy()

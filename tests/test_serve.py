@@ -61,6 +61,9 @@ class FakeCtx:
     def sync(self):
         pass
 
+    def admission(self, on):
+        self.log.append(("admission", bool(on)))
+
     def step_times(self, systems):
         cap = 256
         t = np.zeros((len(systems), cap))
