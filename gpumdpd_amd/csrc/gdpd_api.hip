@@ -1389,8 +1389,12 @@ int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const do
         for (int d = 0; d < 3; ++d) {
             double x = pos[3 * (size_t)k + d];
             if (!std::isfinite(x)) return fail(ctx, GDPD_ERR_NAN, "non-finite position of atom %d", id[k]);
-            // system/read_data.cu:191-197
+            // system/read_data.cu:191-197; a position more than 16 box lengths
+            // out is refused (the reference's one-box-at-a-time loop would run
+            // for as many iterations as the position is box lengths away)
             const double lo = h.lo[d], hi = h.lo[d] + h.L[d];
+            if (x < lo - 16.0 * h.L[d] || x >= hi + 16.0 * h.L[d])
+                return fail(ctx, GDPD_ERR_ARG, "atom %d lies more than 16 box lengths outside the box", id[k]);
             while (x >= hi || x < lo) x -= (hi - lo) * (double)((x >= hi) - (x < lo));
             X[(size_t)d * N + k] = x;
             if (vel) V[(size_t)d * N + k] = vel[3 * (size_t)k + d];

@@ -308,9 +308,14 @@ void read_data(System &s, const std::string &path)
                 s.h_type[k] = (int)ty;
                 if (molecular) s.h_mol_id[k] = (int)mol;
                 for (int d = 0; d < 3; ++d) {
-                    // Read_data::adjust_coordinate (system/read_data.cu:191-197)
+                    // Read_data::adjust_coordinate (system/read_data.cu:191-197); a
+                    // coordinate more than 16 box lengths out is refused rather
+                    // than walked back one box length at a time
                     double v = x[d];
-                    while (v >= hi[d] || v < lo[d]) v -= (hi[d] - lo[d]) * ((v >= hi[d]) - (v < lo[d]));
+                    const double len = hi[d] - lo[d];
+                    if (v < lo[d] - 16.0 * len || v >= hi[d] + 16.0 * len)
+                        throw Error("data file: atom coordinate more than 16 box lengths outside the box: " + r);
+                    while (v >= hi[d] || v < lo[d]) v -= len * ((v >= hi[d]) - (v < lo[d]));
                     s.h_pos[3 * k + d] = v;
                 }
             }

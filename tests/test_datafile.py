@@ -3,6 +3,7 @@ Python writer and reader, the reference's error cases, and a whole deck
 (run.in + data file) run through the C++ shells on the GPU, compared with the
 same system run through the C ABI directly."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -96,6 +97,8 @@ def test_roundtrip_angles(tmp_path):
     (lambda t: t.replace("\n2 1 ", "\n1 1 ", 1), "duplicate atom id"),
     (lambda t: t.replace("\n1 1 ", "\n0 1 ", 1), "Atom ID cannot be 0"),
     (lambda t: t.replace(" atom types", " atom types\n3 angles"), "angles but atom_style"),
+    # a coordinate far outside the box is refused, not walked back box by box
+    (lambda t: re.sub(r"\n(1 1 )\S+", r"\n\g<1>99999999999", t, count=1), "more than 16 box lengths"),
 ])
 def test_reference_error_cases(tmp_path, edit, message):
     cfg = configs.scaled_dpd(300, seed=5)

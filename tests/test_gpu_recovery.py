@@ -205,3 +205,26 @@ def test_neigh_check_after_motion(ctx):
         assert np.array_equal(nb, onb)
         assert s.neigh_check() == 0
         s.destroy()
+
+
+def test_upload_refuses_positions_far_outside_the_box(ctx):
+    """Uploaded positions are wrapped into the box one box length at a time, as
+    the reference does (system/read_data.cu:191-197); one more than 16 box
+    lengths out is refused instead of looping for that many box lengths."""
+    from gpumdpd_amd import _lib
+    cfg = configs.scaled_dpd(2000, seed=71)
+    pos = np.array(cfg["pos"], copy=True)
+    pos[5, 1] = 1e11
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F32, upload=False)
+    print("case: far-out position at upload", flush=True)
+    with pytest.raises(_lib.GdpdError) as e:
+        s.upload(cfg["id"], cfg["type"], pos, cfg["vel"], cfg["masses"])
+    assert "16 box lengths" in str(e.value)
+    # three box lengths out is wrapped back as the reference's loop does
+    L = np.asarray(cfg["hi"]) - np.asarray(cfg["lo"])
+    pos = np.array(cfg["pos"], copy=True)
+    pos[5] += 3.0 * L
+    s.upload(cfg["id"], cfg["type"], pos, cfg["vel"], cfg["masses"])
+    got = s.download()["pos"][5]
+    assert np.allclose(got, cfg["pos"][5], atol=1e-12)
+    s.destroy()
