@@ -190,3 +190,36 @@ def test_deck_dump_and_thermo_files(tmp_path, wrapped):
     assert np.allclose(rows[:, 1], ref_rows[:, 0], atol=2e-6)
     assert np.allclose(rows[:, 2], ref_rows[:, 1], atol=2e-6)
     assert np.allclose(rows[:, 3:], ref_rows[:, 3:10], atol=2e-6)
+
+
+def test_mutated_files_are_read_or_refused(tmp_path):
+    """Randomly damaged data files (fields replaced by junk, huge or non-finite
+    numbers, lines dropped, added or cut off) are either read or refused with
+    an error; the reader never hangs or crashes."""
+    import random
+    rng = random.Random(19)
+    cfg = configs.scaled_dpd(200, seed=9)
+    src = str(tmp_path / "base.data")
+    datafile.write_data(cfg, src)
+    base = open(src).read().splitlines()
+    junk = ["", "-1", "0", "1e400", "nan", "abc", "99999999999", "3.5", "2147483648", "#", "Atoms", "Masses"]
+    for it in range(150):
+        lines = list(base)
+        for _ in range(rng.randint(1, 3)):
+            op = rng.random()
+            if op < 0.4:
+                i = rng.randrange(len(lines))
+                f = lines[i].split()
+                if f:
+                    f[rng.randrange(len(f))] = rng.choice(junk)
+                    lines[i] = " ".join(f)
+            elif op < 0.6:
+                del lines[rng.randrange(len(lines))]
+            elif op < 0.8:
+                lines.insert(rng.randrange(len(lines) + 1), rng.choice(junk + ["1 1 1 0 0 0", "5 atoms"]))
+            else:
+                lines = lines[:rng.randrange(1, len(lines))]
+        path = str(tmp_path / "m.data")
+        open(path, "w").write("\n".join(lines) + "\n")
+        r = roundtrip("atomic", path, str(tmp_path / "out.data"))
+        assert r.returncode in (0, 1), (it, r.returncode, r.stderr[-300:])
