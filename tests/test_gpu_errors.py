@@ -130,3 +130,22 @@ def test_box_too_small_for_the_stencil(ctx):
     print("case: box under 3 cells", flush=True)
     refused(ARG, "fewer than 3 cells", s.run, 2)
     s.destroy()
+
+
+@pytest.mark.parametrize("bad", ["fast", "nan"])
+def test_runaway_bead_is_reported(ctx, bad):
+    """A bead that would move more than 16 box lengths in one step, or whose
+    velocity is not finite, stops the run with GDPD_ERR_NAN instead of being
+    binned out of range (the kernels put it back at the box corner and flag
+    the system)."""
+    cfg = configs.scaled_dpd(2000, seed=6)
+    vel = np.array(cfg["vel"], copy=True)
+    vel[11] = [1.0e5, 0.0, 0.0] if bad == "fast" else [np.nan, 0.0, 0.0]
+    print(f"case: runaway bead {bad}", flush=True)
+    s = g.system_from_config(ctx, dict(cfg, vel=vel), precision=g.PREC_F32)
+    refused(NAN, "", s.run, 5)
+    s.destroy()
+    # the context goes on working
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F32)
+    s.run(5)
+    s.destroy()
