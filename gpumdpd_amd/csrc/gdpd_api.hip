@@ -451,11 +451,7 @@ int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
     T.bl_dpd.perbead = T.bl_mdpd.perbead = ((mode & kModeForceOnly) && (mode & kModeLogForce)) ? 1 : 0;
     const bool timed = ctx->timing;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (T.n_mdpd) {
-        launch_density(ctx->d_reps, T.bricks_mdpd, T.n_mdpd, T.bl_mdpd, T.precision, ctx->stream);
-        // the force pass stages complete densities, halo beads' included
-        launch_ghost_fold(ctx->d_reps, T.beads, T.n_beads, 1, ctx->stream);
-    }
+    if (T.n_mdpd) launch_density(ctx->d_reps, T.bricks_mdpd, T.n_mdpd, T.bl_mdpd, T.precision, ctx->stream);
     if (timed) {
         e0 = next_event(ctx);
         e1 = next_event(ctx);
@@ -468,9 +464,6 @@ int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
         HIPCHK(hipEventRecord(e1, ctx->stream));
         ctx->force_launches++;
     }
-    // the hook sequence reads the force array before any integrate: add the
-    // cross-brick records now (the fused integrate adds them itself)
-    if (mode & kModeForceOnly) launch_ghost_fold(ctx->d_reps, T.beads, T.n_beads, 0, ctx->stream);
     return check_launch(ctx);
 }
 
@@ -532,11 +525,6 @@ int choose_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cell_start)
         for (int rx = -1; rx <= ext[0]; ++rx)
             for (int ry = -1; ry <= ext[1]; ++ry)
                 for (int rz = -1; rz <= ext[2]; ++rz) {
-                    // halo cells in negative directions are not staged (ghost_dir)
-                    const int sx = rx < 0 ? -1 : (rx == ext[0] ? 1 : 0);
-                    const int sy = ry < 0 ? -1 : (ry == ext[1] ? 1 : 0);
-                    const int sz = rz < 0 ? -1 : (rz == ext[2] ? 1 : 0);
-                    if ((sx | sy | sz) != 0 && !ghost_dir(sx, sy, sz)) continue;
                     const int gx = (c0[0] + rx + h.nc[0]) % h.nc[0];
                     const int gy = (c0[1] + ry + h.nc[1]) % h.nc[1];
                     const int gz = (c0[2] + rz + h.nc[2]) % h.nc[2];
@@ -582,18 +570,11 @@ int fit_capacities(gdpd_ctx *ctx, Sys *s, const std::vector<int> &cs_host, int s
     for (int guard = 0; guard < 16; ++guard) {
         BrickLaunch b = brick_params_of(s);
         if (b.lds_bytes(h.precision, s->style) <= (size_t)ctx->max_lds) return GDPD_OK;
-        // shrink the longest edge that can shrink: y and z edges keep every
-        // brick at least two cells thick (ghost_edge)
-        int dmax = -1, e_new = 0;
-        for (int d = 0; d < 3; ++d) {
-            const int e = d == 0 ? h.bdim[0] - 1 : ghost_edge_at_most(h.nc[d], h.bdim[d] - 1);
-            if (e >= 1 && (dmax < 0 || h.bdim[d] > h.bdim[dmax])) {
-                dmax = d;
-                e_new = e;
-            }
-        }
-        if (dmax < 0) return fail(ctx, GDPD_ERR_OVERFLOW, "bead density too high for LDS staging");
-        h.bdim[dmax] = e_new;
+        int dmax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (h.bdim[d] > h.bdim[dmax]) dmax = d;
+        if (h.bdim[dmax] <= 1) return fail(ctx, GDPD_ERR_OVERFLOW, "bead density too high for LDS staging");
+        h.bdim[dmax]--;
         for (int d = 0; d < 3; ++d) h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
         h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
         if (ctx->admitting) ctx->graveyard.push_back(h.thermo_part);
@@ -671,10 +652,6 @@ int prepare(gdpd_ctx *ctx, Sys *s)
             h.bdim[dmax]--;
         }
     }
-    // every brick at least two cells thick in y and z, so each bead receives at
-    // most one cross-brick record per pattern (a tuned shape is moved too)
-    h.bdim[1] = ghost_edge(h.nc[1], h.bdim[1]);
-    h.bdim[2] = ghost_edge(h.nc[2], h.bdim[2]);
     for (int d = 0; d < 3; ++d) h.nbk[d] = (h.nc[d] + h.bdim[d] - 1) / h.bdim[d];
     h.nbrick = h.nbk[0] * h.nbk[1] * h.nbk[2];
     if ((h.bdim[0] + 2) * (h.bdim[1] + 2) * (h.bdim[2] + 2) > 4 * kBlock)
@@ -1267,9 +1244,8 @@ int64_t gdpd_system_bytes(const gdpd_system_desc *d, int n_bonds)
     double vol = 1.0;
     for (int k = 0; k < 3; ++k) vol *= (d->hi[k] - d->lo[k]);
     const int64_t ncell = (int64_t)std::max(1.0, vol / std::max(d->global_cut * d->global_cut * d->global_cut, 1e-3));
-    // state x2 (96) + force/pe/bpe/ape/vir/rho (104) + ints and image word (44) + type,
-    // cross-brick records (168) and their mask
-    return N * 418 + ncell * 8 + (int64_t)n_bonds * 16 + (1 << 20);
+    // state x2 (96) + force/pe/bpe/ape/vir/rho (104) + ints and image word (44) + type
+    return N * 249 + ncell * 8 + (int64_t)n_bonds * 16 + (1 << 20);
 }
 
 int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
@@ -1336,8 +1312,6 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     ALLOC(h.angle_start, (size_t)N + 1);
     ALLOC(h.vir, 6 * (size_t)N);
     ALLOC(h.rho, N);
-    ALLOC(h.ghost, (size_t)kGhostPatterns * 3 * N);
-    ALLOC(h.gmask, N);
     ALLOC(h.iscratch, 4 * (size_t)N);
     ALLOC(h.state, 1);
     ALLOC(h.step_ts, kStepTsCap);
@@ -1363,8 +1337,6 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     HIPCHK(hipMemsetAsync(h.angle_start, 0, sizeof(int) * ((size_t)N + 1), ctx->stream));
     HIPCHK(hipMemsetAsync(h.vir, 0, sizeof(double) * 6 * (size_t)N, ctx->stream));
     HIPCHK(hipMemsetAsync(h.rho, 0, sizeof(double) * (size_t)N, ctx->stream));
-    HIPCHK(hipMemsetAsync(h.ghost, 0, sizeof(double) * kGhostPatterns * 3 * (size_t)N, ctx->stream));
-    HIPCHK(hipMemsetAsync(h.gmask, 0, (size_t)N, ctx->stream));
     RepState st{};
     HIPCHK(hipMemcpyAsync(h.state, &st, sizeof(st), hipMemcpyHostToDevice, ctx->stream));
     if ((rc = push_desc(ctx, id))) return rc;

@@ -111,10 +111,6 @@ struct RepDev {
     double *pe, *vir;          // N and SoA 6N (log steps)
     double *bpe;               // N: per-bead half of its bonds' energy (log steps)
     double *rho;               // N (mDPD)
-    // cross-brick shares (ghost_* below): [kGhostPatterns][N][3] records and
-    // the per-slot mask of the patterns that hold one this step
-    double *ghost;
-    unsigned char *gmask;
     int nbonds;
     int *bond_start, *bond_partner;  // CSR by id-1: partner ids (1-based)
     unsigned char *bond_type;        // per CSR entry, 0-based
@@ -174,70 +170,6 @@ inline int brick_run_entries(int ex, int ey, int ez)
         for (int ry = 1; ry <= ey; ++ry)
             for (int rz = 1; rz <= ez; ++rz) n += cell_run_cap(ex, ey, ez, rx, ry, rz);
     return n;
-}
-
-// Pairs across a brick face are evaluated once.  Besides its own cells, a
-// brick stages only the halo cells in the 13 neighbour directions
-// s = (sx, sy, sz) that are lexicographically positive (sx > 0, or sx = 0 and
-// sy > 0, or sx = sy = 0 and sz > 0), and evaluates each pair of an owned bead
-// with a staged halo bead.  -s is then never staged from the other side, so
-// every pair has one evaluating brick.  The halo bead's share of its pairs is
-// summed in that brick and stored as a record at (pattern, slot), with
-// pattern = |sx| + 2 |sy| + 4 |sz| - 1 in 0..6; the bead's own brick marks the
-// patterns its cell can receive (ghost_mask) and the records are added to the
-// bead's sum.  sx is never -1, and with every brick at least two cells thick
-// in y and z (ghost_edge) a cell is on at most one y face and one z face, so
-// a bead receives at most one record per pattern.
-constexpr int kGhostPatterns = 7;
-
-__host__ __device__ inline bool ghost_dir(int sx, int sy, int sz)
-{
-    return sx > 0 || (sx == 0 && (sy > 0 || (sy == 0 && sz > 0)));
-}
-__host__ __device__ inline int ghost_pattern(int sx, int sy, int sz)
-{
-    return (sx != 0 ? 1 : 0) + (sy != 0 ? 2 : 0) + (sz != 0 ? 4 : 0) - 1;
-}
-// Patterns of the records an owned cell receives: lx, the cell is on its
-// brick's low x face; ly / hy and lz / hz, on its low / high y and z faces.
-__host__ __device__ inline unsigned ghost_mask(bool lx, bool ly, bool hy, bool lz, bool hz)
-{
-    const bool ay = ly || hy, az = lz || hz;
-    unsigned m = 0;
-    if (lx) {
-        m |= 1u << 0;                  // (1, 0, 0)
-        if (ay) m |= 1u << 2;          // (1, +-1, 0)
-        if (az) m |= 1u << 4;          // (1, 0, +-1)
-        if (ay && az) m |= 1u << 6;    // (1, +-1, +-1)
-    }
-    if (ly) {
-        m |= 1u << 1;                  // (0, 1, 0)
-        if (az) m |= 1u << 5;          // (0, 1, +-1)
-    }
-    if (lz) m |= 1u << 3;              // (0, 0, 1)
-    return m;
-}
-// A y or z brick edge of b cells over nc cells leaves every brick at least two
-// cells thick: b >= 2 and no last brick of one cell.  Returns b, or the
-// nearest edge (smaller first) that does; nc >= 3 always has one.
-inline int ghost_edge(int nc, int b)
-{
-    auto ok = [&](int e) { return e >= 2 && (e >= nc || nc % e != 1); };
-    b = b < 1 ? 1 : (b > nc ? nc : b);
-    if (ok(b)) return b;
-    for (int k = 1; k <= nc; ++k) {
-        if (b - k >= 2 && ok(b - k)) return b - k;
-        if (b + k <= nc && ok(b + k)) return b + k;
-    }
-    return nc;
-}
-
-// The largest y or z brick edge e <= b that ghost_edge accepts, or 0 if none.
-inline int ghost_edge_at_most(int nc, int b)
-{
-    for (int e = b < nc ? b : nc; e >= 2; --e)
-        if (e >= nc || nc % e != 1) return e;
-    return 0;
 }
 
 // One workgroup's work: replica index and the local chunk/brick index.

@@ -29,9 +29,6 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s);
 void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
                   int precision, int style, int mode, hipStream_t s);
-// adds the cross-brick records to the force array (what 0) or to the density
-// of mDPD replicas (what 1); items: rep, chunk of kBlock slots
-void launch_ghost_fold(const RepDev *reps, const WorkItem *items, int nitems, int what, hipStream_t s);
 // bonds + integration + re-binning, one thread per slot; topo: 0 no replica has
 // bonds or angles, 1 bonds only, 2 angles too (selects the kernel instance)
 void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, int topo, hipStream_t s);

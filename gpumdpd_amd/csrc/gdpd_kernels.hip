@@ -359,21 +359,19 @@ __global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restri
 // brick kernels: LDS staging, per-wave pair queue, fixed-point accumulation
 // ---------------------------------------------------------------------------
 //
-// One 256-thread workgroup per brick of cells.  The brick and the half of its
-// one-cell halo in the 13 positive neighbour directions (ghost_dir) are staged
-// into LDS (positions relative to the brick corner, periodic images applied
-// while staging).  Each pair of beads is evaluated once in the whole system:
+// One 256-thread workgroup per brick of cells.  The brick and a one-cell halo
+// are staged into LDS (positions relative to the brick corner, periodic images
+// applied while staging).  Each pair of beads is evaluated once inside a brick:
 // an owned bead scans only the staged beads after it in LDS order plus every
-// staged halo bead, so a pair of owned beads is found by its lower bead alone,
-// and a pair with a bead of a neighbouring brick by the brick that stages the
-// other as halo.  The halo bead's share is summed like an owned bead's and
-// stored as a record for its own brick (ghost_mask).  Candidate tests run as
-// wave-uniform loops that append in-range pairs to a per-wave queue (ballot +
-// mbcnt); full queues are evaluated with every lane busy.  Per-bead sums are
-// 64-bit fixed point in LDS: integer additions are exact and commute, so the
-// result does not depend on the order in which waves and lanes deliver their
-// pairs, nor on the batch or the slot layout.  Per-pair arithmetic follows
-// force/pair_dpd.cu:296-399 and force/pair_mdpd.cu:219-300, 403-517.
+// halo bead, so a pair of owned beads is found by its lower bead alone and a
+// pair with a halo bead once per brick (the neighbouring brick evaluates its
+// own side).  Candidate tests run as wave-uniform loops that append in-range
+// pairs to a per-wave queue (ballot + mbcnt); full queues are evaluated with
+// every lane busy.  Per-bead sums are 64-bit fixed point in LDS: integer
+// additions are exact and commute, so the result does not depend on the order
+// in which waves and lanes deliver their pairs, nor on the batch or the slot
+// layout.  Per-pair arithmetic follows force/pair_dpd.cu:296-399 and
+// force/pair_mdpd.cu:219-300, 403-517.
 
 struct BrickParams {
     int rmax, smax, omax, nrun, ntypes, qw, perbead, style;
@@ -388,10 +386,6 @@ template <> struct Vec4T<double> { using type = double4; };
 // type is kept zero-based, as it indexes the pair coefficient table.
 constexpr int kIdBits = 27;
 constexpr int kIdMask = (1 << kIdBits) - 1;
-// A halo bead's sinfo word: its slot (N < 2^27) and its record pattern above.
-constexpr int kSlotMask = kIdMask;
-__device__ __forceinline__ int halo_slot(int si) { return si & kSlotMask; }
-__device__ __forceinline__ int halo_pattern(int si) { return (int)((unsigned)si >> kIdBits); }
 
 __host__ __device__ __forceinline__ int stage_word(int id, int type, bool halo)
 {
@@ -429,13 +423,13 @@ template <typename real> struct BrickLds {
     PairCoef<real> *coef;
     double *red;               // block sums (in the queue's space, after the pair phase)
     real4 *pos;                // staged beads: xyz relative to the brick, w = stage word
-    int *sinfo;                // staged beads: owner index (owned), or slot | record pattern << 27 (halo)
+    int *sinfo;                // staged beads: owner index (owned) or slot (halo)
     real *rh;                  // staged density (mDPD force pass)
     int *c_start, *sigcnt, *wbuf, *oinfo;
     unsigned *runs;            // per owned cell: own-column end, candidate runs (js | je << 16), 0
     int *cinfo, *gstart, *cellpre, *rbase; // staging only (in the sums' space)
     real4 *vown;               // owned beads: vxyz, w = slot bits
-    long long *facc;           // fixed-point sums per staged bead (3 per bead, 1 in the density pass)
+    long long *facc;           // fixed-point sums per owned bead
     unsigned *queue;           // per wave: li | lj << 16
     unsigned short *cellof;    // staging only: region cell of each staged index (in the queues' space)
 };
@@ -443,11 +437,10 @@ template <typename real> struct BrickLds {
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
 // LDS layout as byte offsets; pointers are rebuilt inline from the __shared__
-// base so they stay in the LDS address space.  Halo beads keep a position, a
-// word and their fixed-point sums (44 bytes in FP32 mode); velocities are
-// staged for owned beads alone, and a pair with a halo bead reads the halo
-// velocity from HBM.  This keeps a 4x4x5-cell brick at rho = 3 under 40 KB,
-// four workgroups per CU.
+// base so they stay in the LDS address space.  Halo beads keep only a position
+// and a word (20 bytes in FP32 mode); velocities are staged for owned beads
+// alone, and a pair with a halo bead reads the halo velocity from HBM.  This
+// keeps a 4x4x5-cell brick at rho = 3 under 40 KB, four workgroups per CU.
 struct LdsOffsets {
     unsigned coef, red, pos, sinfo, rh, c_start, cinfo, gstart, cellpre, rbase, sigcnt, wbuf, oinfo, runs, vown,
         facc, queue, cellof, total;
@@ -477,7 +470,7 @@ __host__ __device__ __forceinline__ LdsOffsets lds_offsets(const BrickParams &p)
     o.gstart = (unsigned)(off + sizeof(int) * p.rmax);
     o.cellpre = (unsigned)(off + 2 * sizeof(int) * p.rmax);
     o.rbase = (unsigned)(off + 3 * sizeof(int) * p.rmax);
-    GDPD_TAKE(facc, (3 * 8 * (size_t)p.smax > 4 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.smax : 4 * 4 * (size_t)p.rmax)
+    GDPD_TAKE(facc, (3 * 8 * (size_t)p.omax > 4 * 4 * (size_t)p.rmax) ? 3 * 8 * (size_t)p.omax : 4 * 4 * (size_t)p.rmax)
     // the pair queues; the region cell of each staged index uses the same
     // space while the brick is staged, the block sums after the pair phase
     o.red = (unsigned)off;
@@ -571,9 +564,6 @@ __device__ __forceinline__ int wrap_code(int c, int n, int &code)
     code = 1;
     return c;
 }
-
-// side of a region cell coordinate r in [0, n): -1 low halo, +1 high halo, 0 owned
-__device__ __forceinline__ int region_side(int r, int n) { return r == 0 ? -1 : (r == n - 1 ? 1 : 0); }
 
 __device__ __forceinline__ float id_to_real(int id, float) { return __int_as_float(id); }
 __device__ __forceinline__ double id_to_real(int id, double) { return __longlong_as_double((long long)(unsigned)id); }
@@ -715,15 +705,7 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         const int gz = wrap_code(g.c0[2] - 1 + rz, R.nc[2], wz);
         const int gc = (gx * R.nc[1] + gy) * R.nc[2] + gz;
         const auto *cs = gmem(R.cell_start);
-        // halo cells in the negative directions are left empty: the brick on
-        // that side evaluates their pairs with this brick's beads
-        const int sx = region_side(rx, g.RX), sy = region_side(ry, g.RY), sz = region_side(rz, g.RZ);
-        const bool staged = (sx | sy | sz) == 0 || ghost_dir(sx, sy, sz);
-        int a = 0, b = 0;
-        if (staged) {
-            a = cs[gc];
-            b = cs[gc + 1];
-        }
+        const int a = cs[gc], b = cs[gc + 1];
         L.cinfo[rc] = rx | (ry << 8) | (rz << 16) | (wx << 24) | (wy << 26) | (wz << 28);
         L.gstart[rc] = a;
         L.c_start[rc + 1] = b - a;
@@ -889,19 +871,14 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
                 q.z = (real)vzs[u];
                 q.w = id_to_real(sl[u], (real)0);
                 L.vown[o] = q;
-                // the records of neighbouring bricks this bead's cell receives
-                gmemw(R.gmask)[sl[u]] = (unsigned char)ghost_mask(rx == 1, ry == 1, ry == g.ext[1], rz == 1,
-                                                                  rz == g.ext[2]);
             } else {
-                const int pat = ghost_pattern(region_side(rx, g.RX), region_side(ry, g.RY), region_side(rz, g.RZ));
-                L.sinfo[k] = sl[u] | (pat << kIdBits);
+                L.sinfo[k] = sl[u];
             }
             if (need_rho) L.rh[k] = (real)grho[sl[u]];
         }
     }
     __syncthreads();  // cinfo, gstart and cellpre (aliased by facc) are dead from here
-    const int nsum = (need_vel ? 3 : 1) * S;
-    for (int k = t; k < nsum; k += kBlock) L.facc[k] = 0;
+    for (int k = t; k < 3 * O; k += kBlock) L.facc[k] = 0;
     __syncthreads();
     return O;
 }
@@ -979,11 +956,9 @@ __device__ __forceinline__ int lanes_below(unsigned long long m)
 }
 
 // Thermo partial sums per thread: 0 ke, 1 pair pe, 2 bond pe, 3..8 pressure tensor.
-// Bead i of a queued pair is always owned; j is owned or halo, and both get
-// their share in their per-staged-bead sums.  A halo j's velocity is read from
-// HBM at its slot and rounded to the working precision exactly as staging
-// rounds an owned bead's.  Each pair is evaluated once in the system, so both
-// beads' energy and virial shares go into the brick's sums.
+// Bead i of a queued pair is always owned; j is owned or halo.  A halo j's
+// velocity is read from HBM at its slot and rounded to the working precision
+// exactly as staging rounds an owned bead's.
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real> &L, const unsigned *q, int qn,
                                            int nt, int cur, uint32_t seed_step, bool log, bool perbead,
@@ -1000,7 +975,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         const int li = ent & 0xFFFF, lj = ent >> 16;
         const real4 pi = L.pos[li], pj = L.pos[lj];
         const int wi = real_to_id(pi.w), wj = real_to_id(pj.w);
-        const int oi = L.sinfo[li], sj = L.sinfo[lj];  // sj: owner index of an owned j, slot word of a halo j
+        const int oi = L.sinfo[li], sj = L.sinfo[lj];  // sj: owner index of an owned j, slot of a halo j
         const bool jown = !word_halo(wj);
         const int oj = sj;
         const real dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -1017,8 +992,8 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             real wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
             wd = wd > (real)0 ? wd : (real)0;
             const long long w = to_fix<real>((double)(C.rhofac * wd * wd));
-            lds_add(&L.facc[li], w);
-            lds_add(&L.facc[lj], w);
+            lds_add(&L.facc[oi], w);
+            if (jown) lds_add(&L.facc[oj], w);
             continue;
         }
         // force/pair_dpd.cu: pairs inside rc with r >= 1e-10
@@ -1026,8 +1001,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         // the HBM loads of a halo partner's velocity are issued first and used
         // after the hash, so their latency overlaps it; lanes with an owned
         // partner load slot 0 (one shared address) and ignore the value
-        const int hs = halo_slot(sj);
-        const int sv = (!jown && (unsigned)hs < (unsigned)N) ? hs : 0;
+        const int sv = (!jown && (unsigned)sj < (unsigned)N) ? sj : 0;
         const double gvx = V[sv], gvy = V[N + sv], gvz = V[2 * N + sv];
         real r, rinv;
         root_and_inverse(r2, r, rinv);
@@ -1057,14 +1031,16 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
         const real fps = fpair * (real)FixScale<real>::s;
         const long long qx = fix_scaled<real>(dx * fps), qy = fix_scaled<real>(dy * fps),
                         qz = fix_scaled<real>(dz * fps);
-        long long *fi = lds_elem(L.facc, (unsigned)li, 3u * sizeof(long long));
-        long long *fj = lds_elem(L.facc, (unsigned)lj, 3u * sizeof(long long));
+        long long *fi = lds_elem(L.facc, (unsigned)oi, 3u * sizeof(long long));
         lds_add(fi + 0, qx);
         lds_add(fi + 1, qy);
         lds_add(fi + 2, qz);
-        lds_sub(fj + 0, qx);
-        lds_sub(fj + 1, qy);
-        lds_sub(fj + 2, qz);
+        if (jown) {
+            long long *fj = lds_elem(L.facc, (unsigned)oj, 3u * sizeof(long long));
+            lds_sub(fj + 0, qx);
+            lds_sub(fj + 1, qy);
+            lds_sub(fj + 2, qz);
+        }
         if (log) {
             // per-bead shares as in force/pair_dpd.cu:366-398 and pair_mdpd.cu:479-516
             real pe_i, pe_j;
@@ -1080,19 +1056,22 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             const real w6[6] = {(real)0.5 * dx * ffx, (real)0.5 * dx * ffy, (real)0.5 * dx * ffz,
                                 (real)0.5 * dy * ffy, (real)0.5 * dy * ffz, (real)0.5 * dz * ffz};
             // per-pair fixed point: the brick sums do not depend on pair order
-            acc[1] += thermo_fix((double)pe_i) + thermo_fix((double)pe_j);
+            acc[1] += thermo_fix((double)pe_i) + (jown ? thermo_fix((double)pe_j) : 0ll);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc[3 + c] += 2 * thermo_fix((double)w6[c]);
+            for (int c = 0; c < 6; ++c) {
+                const long long wq = thermo_fix((double)w6[c]);
+                acc[3 + c] += jown ? 2 * wq : wq;
+            }
             if (perbead) {
                 // hook-path log step: per-bead fixed-point sums in the pe/virial arrays
-                const int si = real_to_id(vi.w), sjs = jown ? real_to_id(L.vown[oj].w) : hs;
+                const int si = real_to_id(vi.w), sjs = jown ? real_to_id(L.vown[oj].w) : 0;
                 atomicAdd((unsigned long long *)&R.pe[si], (unsigned long long)to_fix<real>((double)pe_i));
-                atomicAdd((unsigned long long *)&R.pe[sjs], (unsigned long long)to_fix<real>((double)pe_j));
+                if (jown) atomicAdd((unsigned long long *)&R.pe[sjs], (unsigned long long)to_fix<real>((double)pe_j));
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     const unsigned long long wq = (unsigned long long)to_fix<real>((double)w6[c]);
                     atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + si], wq);
-                    atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + sjs], wq);
+                    if (jown) atomicAdd((unsigned long long *)&R.vir[(size_t)c * N + sjs], wq);
                 }
             }
         }
@@ -1268,15 +1247,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
     brick_pairs<real, 2, true>(R, L, bp, O, R.ntypes, (real)R.rdmax2, cur, 0u, false, acc);
     for (int o = threadIdx.x; o < O; o += kBlock)
-        R.rho[real_to_id(L.vown[o].w)] = (double)L.facc[L.oinfo[o] & 0xFFFF] * FixScale<real>::inv;
-    // halo beads' shares, as records for their own bricks (component 0)
-    const int S = L.c_start[g.nreg];
-    auto *G = gmemw(R.ghost);
-    for (int k = threadIdx.x; k < S; k += kBlock) {
-        if (!word_halo(real_to_id(L.pos[k].w))) continue;
-        const int si = L.sinfo[k];
-        G[((size_t)halo_pattern(si) * R.N + halo_slot(si)) * 3] = (double)L.facc[k] * FixScale<real>::inv;
-    }
+        R.rho[real_to_id(L.vown[o].w)] = (double)L.facc[o] * FixScale<real>::inv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1314,27 +1285,12 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     auto *F = gmemw(R.force);
     for (int o = t; o < O; o += kBlock) {
         const int slot = real_to_id(L.vown[o].w);
-        const long long *fs = L.facc + 3 * (L.oinfo[o] & 0xFFFF);
-        F[slot] = (double)fs[0] * FixScale<real>::inv;
-        F[N + slot] = (double)fs[1] * FixScale<real>::inv;
-        F[2 * N + slot] = (double)fs[2] * FixScale<real>::inv;
+        F[slot] = (double)L.facc[3 * o + 0] * FixScale<real>::inv;
+        F[N + slot] = (double)L.facc[3 * o + 1] * FixScale<real>::inv;
+        F[2 * N + slot] = (double)L.facc[3 * o + 2] * FixScale<real>::inv;
         if (log && bp.perbead) {
             R.bpe[slot] = 0.0;
             R.ape[slot] = 0.0;
-        }
-    }
-    // halo beads' shares, as records for their own bricks
-    {
-        const int S = L.c_start[g.nreg];
-        auto *G = gmemw(R.ghost);
-        for (int k = t; k < S; k += kBlock) {
-            if (!word_halo(real_to_id(L.pos[k].w))) continue;
-            const int si = L.sinfo[k];
-            auto *rec = G + ((size_t)halo_pattern(si) * N + halo_slot(si)) * 3;
-            const long long *fs = L.facc + 3 * k;
-            rec[0] = (double)fs[0] * FixScale<real>::inv;
-            rec[1] = (double)fs[1] * FixScale<real>::inv;
-            rec[2] = (double)fs[2] * FixScale<real>::inv;
         }
     }
     if (log && !bp.perbead) {
@@ -1403,54 +1359,6 @@ __device__ __forceinline__ void angle_on_bead(const RepDev &R, const double *X, 
 // buffer.
 // ---------------------------------------------------------------------------
 
-// Adds the records of the patterns in gm to (fx, fy, fz), in pattern order.  In
-// FP32 mode every term is a multiple of 2^-24 below 2^29 in size, so the double
-// sums are exact and equal the fixed-point sum of all the bead's pairs.
-__device__ __forceinline__ void ghost_add3(const RepDev &R, int s, unsigned gm, double &fx, double &fy, double &fz)
-{
-    const auto *G = gmem(R.ghost);
-#pragma unroll
-    for (int p = 0; p < kGhostPatterns; ++p)
-        if ((gm >> p) & 1u) {
-            const auto *rec = G + ((size_t)p * R.N + s) * 3;
-            fx += rec[0];
-            fy += rec[1];
-            fz += rec[2];
-        }
-}
-
-// Folds the records into the force array (what 0) or into the density (what 1,
-// mDPD replicas only, component 0), for the paths that read those arrays
-// before an integrate: the hook sequence, and the mDPD force pass.
-__global__ __launch_bounds__(kBlock) void k_ghost_fold(const RepDev *__restrict__ reps,
-                                                       const WorkItem *__restrict__ items, int nitems, int what)
-{
-    const WorkItem it = items[xcd_item(nitems)];
-    const RepDev &R = reps[it.rep];
-    if (halted(R)) return;
-    if (what == 1 && R.style != 2) return;
-    const int N = R.N;
-    const int s = it.local * kBlock + threadIdx.x;
-    if (s >= N) return;
-    const unsigned gm = gmem(R.gmask)[s];
-    if (!gm) return;
-    if (what == 0) {
-        auto *F = gmemw(R.force);
-        double fx = F[s], fy = F[N + s], fz = F[2 * N + s];
-        ghost_add3(R, s, gm, fx, fy, fz);
-        F[s] = fx;
-        F[N + s] = fy;
-        F[2 * N + s] = fz;
-    } else {
-        const auto *G = gmem(R.ghost);
-        double r = R.rho[s];
-#pragma unroll
-        for (int p = 0; p < kGhostPatterns; ++p)
-            if ((gm >> p) & 1u) r += G[((size_t)p * N + s) * 3];
-        R.rho[s] = r;
-    }
-}
-
 // TOPO: 0 no replica of the launch has bonds or angles, 1 bonds only, 2 angles
 // too.  Code a launch cannot need is compiled out: the plain-fluid instance
 // holds far fewer registers, so more waves keep HBM busy.
@@ -1479,9 +1387,6 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
         const auto *V = gmem(R.v[cur]);
         const auto *F = gmem(R.force);
         double fx = F[s], fy = F[N + s], fz = F[2 * N + s];
-        // the shares of pairs that neighbouring bricks evaluated
-        const unsigned gm = gmem(R.gmask)[s];
-        if (gm) ghost_add3(R, s, gm, fx, fy, fz);
         double xg = Xg[s], yg = Xg[N + s], zg = Xg[2 * N + s];
         double bpe = 0.0, bw[6] = {0, 0, 0, 0, 0, 0};
         if (TOPO >= 1 && R.nbonds > 0) {
@@ -1522,7 +1427,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
             fy = fa[1];
             fz = fa[2];
         }
-        if ((gm || R.nbonds > 0 || R.nangles > 0) && (mode & kModeStoreForce)) {
+        if ((R.nbonds > 0 || R.nangles > 0) && (mode & kModeStoreForce)) {
             R.force[s] = fx;
             R.force[N + s] = fy;
             R.force[2 * N + s] = fz;
@@ -2580,10 +2485,6 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
         else
             hipLaunchKernelGGL((k_brick<float, 1>), dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p, mode);
     }
-}
-void launch_ghost_fold(const RepDev *reps, const WorkItem *items, int nitems, int what, hipStream_t s)
-{
-    if (nitems) hipLaunchKernelGGL(k_ghost_fold, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems, what);
 }
 void launch_integrate(const RepDev *reps, const WorkItem *items, int nitems, int mode, int topo, hipStream_t s)
 {
