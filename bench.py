@@ -205,6 +205,7 @@ def main():
     s.advance(args.steps)
     ctx.sync()
     k_ms, k_n, _ = ctx.kernel_timing_get()
+    phases = ctx.kernel_phases()
     ctx.kernel_timing(False)
     s.finish()
 
@@ -242,6 +243,7 @@ def main():
                      "bytes_per_bead": per_kernel, "launches_timed": k_n,
                      "kernel_timing": "HIP events around every k_brick launch of a second, eager pass of the "
                                       "same number of steps on the library stream",
+                     "phases_us_per_wave": phases,
                      "step": {"bytes_per_bead_step": per, "achieved": step_gbs,
                               "frac": step_gbs / HBM_PEAK_GBS}},
     }

@@ -92,6 +92,7 @@ PROTOTYPES = {
     "gdpd_get_tuning": (_I, [_P, _I, _PI]),
     "gdpd_kernel_timing": (_I, [_P, _I]),
     "gdpd_kernel_timing_get": (_I, [_P, _PD, C.POINTER(C.c_int64), _PD]),
+    "gdpd_kernel_phases": (_I, [_P, _PD]),
     "gdpd_sync": (_I, [_P]),
     "gdpd_admission": (_I, [_P, _I]),
     "gdpd_thermo_noalloc": (_I, [_P, _I, _PD]),

@@ -152,6 +152,10 @@ struct gdpd_ctx {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     int64_t force_launches = 0;
+    // phase clocks of the timed force launches: 4 sums per work item and wave
+    unsigned *d_phase = nullptr;
+    size_t phase_items = 0;        // items the buffer holds
+    double phase_wave_launches = 0; // sum over timed launches of their waves
     // batch tables cache
     Tables batch;
     Graph batch_graph;
@@ -456,7 +460,27 @@ int force_launch(gdpd_ctx *ctx, Tables &T, int mode)
         e0 = next_event(ctx);
         e1 = next_event(ctx);
         if (!e0 || !e1) return fail(ctx, GDPD_ERR_HIP, "hipEventCreate failed");
+        // phase clocks: the buffer holds the items of the largest timed launch
+        const size_t items = (size_t)T.n_dpd + (size_t)T.n_mdpd;
+        if (items > ctx->phase_items) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            std::vector<unsigned> keep(ctx->phase_items * 16);
+            if (ctx->d_phase) {
+                HIPCHK(hipMemcpy(keep.data(), ctx->d_phase, keep.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+                (void)hipFree(ctx->d_phase);
+                ctx->d_phase = nullptr;
+            }
+            keep.resize(items * 16, 0u);
+            HIPCHK(hipMalloc((void **)&ctx->d_phase, keep.size() * sizeof(unsigned)));
+            HIPCHK(hipMemcpy(ctx->d_phase, keep.data(), keep.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+            ctx->phase_items = items;
+        }
+        T.bl_dpd.phase = ctx->d_phase;
+        T.bl_mdpd.phase = ctx->d_phase + (size_t)T.n_dpd * 16;
+        ctx->phase_wave_launches += (double)items * (kBlock / 64);
         HIPCHK(hipEventRecord(e0, ctx->stream));
+    } else {
+        T.bl_dpd.phase = T.bl_mdpd.phase = nullptr;
     }
     if (T.n_dpd) launch_brick(ctx->d_reps, T.bricks_dpd, T.n_dpd, T.bl_dpd, T.precision, GDPD_STYLE_DPD, mode, ctx->stream);
     if (T.n_mdpd) launch_brick(ctx->d_reps, T.bricks_mdpd, T.n_mdpd, T.bl_mdpd, T.precision, GDPD_STYLE_MDPD, mode, ctx->stream);
@@ -1195,7 +1219,7 @@ int gdpd_destroy(gdpd_ctx *ctx)
     ctx->batch.release();
     ctx->batch_graph.release();
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
-    void *gb[] = {ctx->d_ids, ctx->d_states, ctx->d_ts, ctx->d_tsteps};
+    void *gb[] = {ctx->d_ids, ctx->d_states, ctx->d_ts, ctx->d_tsteps, ctx->d_phase};
     for (void *q : gb)
         if (q) (void)hipFree(q);
     if (ctx->d_reps) (void)hipFree(ctx->d_reps);
@@ -2391,6 +2415,23 @@ int gdpd_kernel_timing(gdpd_ctx *ctx, int on)
     ctx->timing = on != 0;
     ctx->ev_used = 0;
     ctx->force_launches = 0;
+    ctx->phase_wave_launches = 0;
+    if (ctx->d_phase) HIPCHK(hipMemset(ctx->d_phase, 0, ctx->phase_items * 16 * sizeof(unsigned)));
+    return GDPD_OK;
+}
+
+int gdpd_kernel_phases(gdpd_ctx *ctx, double *us4)
+{
+    if (!ctx || !us4) return GDPD_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double sum[4] = {0, 0, 0, 0};
+    if (ctx->d_phase) {
+        std::vector<unsigned> h(ctx->phase_items * 16);
+        HIPCHK(hipMemcpy(h.data(), ctx->d_phase, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < h.size(); ++k) sum[k & 3] += h[k];
+    }
+    for (int k = 0; k < 4; ++k)
+        us4[k] = ctx->phase_wave_launches > 0 ? sum[k] * ctx->tick_ns * 1e-3 / ctx->phase_wave_launches : 0.0;
     return GDPD_OK;
 }
 

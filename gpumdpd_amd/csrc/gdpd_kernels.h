@@ -12,6 +12,7 @@ struct BrickLaunch {
     int ntypes;    // max types
     int qw = 192;  // pair-queue entries per wave: a flush leaves < 64, one candidate step adds <= 128
     int perbead = 0; // per-bead PE/virial sums (hook path log steps)
+    unsigned *phase = nullptr;  // per item and wave: 4 phase-clock sums (kernel timing)
     size_t lds_bytes(int precision, int style) const;
 };
 

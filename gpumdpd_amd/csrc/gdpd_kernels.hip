@@ -375,7 +375,21 @@ __global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restri
 
 struct BrickParams {
     int rmax, smax, omax, nrun, ntypes, qw, perbead, style;
+    // phase clocks (kernel timing on): per work item and wave, the summed
+    // device-clock ticks of staging, pair work, the closing barrier and the
+    // write-back; null when off
+    unsigned *phase;
 };
+
+// lane 0 of each wave adds its phase durations (t[0..4] device clock) to the
+// work item's four counters; each counter has one writer, so plain adds
+__device__ __forceinline__ void add_phases(const BrickParams &bp, int item, const unsigned long long (&t)[5])
+{
+    if ((threadIdx.x & 63) != 0) return;
+    unsigned *c = bp.phase + ((size_t)item * (kBlock / 64) + (threadIdx.x >> 6)) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] += (unsigned)(t[k + 1] - t[k]);
+}
 
 template <typename real> struct Vec4T;
 template <> struct Vec4T<float> { using type = float4; };
@@ -1093,7 +1107,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
 template <typename real, int STYLE, bool DENSITY>
 __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickLds<real> &L, const BrickParams &bp, int O,
                                             int nt, real cut2, int cur, uint32_t seed_step, bool log,
-                                            long long (&acc)[kThermoParts])
+                                            long long (&acc)[kThermoParts], unsigned long long *t_done = nullptr)
 {
     using real4 = typename Vec4T<real>::type;
     const int t = threadIdx.x, lane = t & 63;
@@ -1216,6 +1230,7 @@ __device__ __forceinline__ void brick_pairs(const RepDev &R, const BrickLds<real
         }
     }
     eval_queue<real, STYLE, DENSITY>(R, L, q, qn, nt, cur, seed_step, log, bp.perbead, acc);
+    if (t_done) *t_done = wall_clock64();
     __syncthreads();
 }
 
@@ -1262,9 +1277,12 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
 {
     extern __shared__ __align__(16) char smem[];
     const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
-    const WorkItem it = items[xcd_item(nitems)];
+    const int item = xcd_item(nitems);
+    const WorkItem it = items[item];
     const RepDev &R = reps[it.rep];
     if (halted(R)) return;
+    unsigned long long tp[5] = {0, 0, 0, 0, 0};
+    if (bp.phase) tp[0] = wall_clock64();
     const int cur = R.state->cur;
     const long long step = R.state->step;
     const bool log = (mode & kModeLogForce) ||
@@ -1281,7 +1299,10 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     long long acc[kThermoParts];
 #pragma unroll
     for (int k = 0; k < kThermoParts; ++k) acc[k] = 0;
-    brick_pairs<real, STYLE, false>(R, L, bp, O, R.ntypes, (real)R.rcmax2, cur, seed_step, log, acc);
+    if (bp.phase) tp[1] = wall_clock64();
+    brick_pairs<real, STYLE, false>(R, L, bp, O, R.ntypes, (real)R.rcmax2, cur, seed_step, log, acc,
+                                    bp.phase ? &tp[2] : nullptr);
+    if (bp.phase) tp[3] = wall_clock64();
     auto *F = gmemw(R.force);
     for (int o = t; o < O; o += kBlock) {
         const int slot = real_to_id(L.vown[o].w);
@@ -1297,6 +1318,10 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
         long long *red = (long long *)L.red, *out = red + 48;
         block_sum_i64<kThermoParts>(acc, red, out);
         if (t < kThermoParts) ((long long *)R.thermo_part)[(size_t)it.local * kThermoParts + t] = out[t];
+    }
+    if (bp.phase) {
+        tp[4] = wall_clock64();
+        add_phases(bp, item, tp);
     }
 }
 
@@ -2425,7 +2450,7 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / 
 
 size_t BrickLaunch::lds_bytes(int precision, int style) const
 {
-    BrickParams p{rmax, smax, omax, nrun, ntypes, qw, perbead, style};
+    BrickParams p{rmax, smax, omax, nrun, ntypes, qw, perbead, style, nullptr};
     return precision == 1 ? lds_offsets<double>(p).total : lds_offsets<float>(p).total;
 }
 
@@ -2461,7 +2486,7 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, 0, 1};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, 0, 1, nullptr};
     const size_t lds = bl.lds_bytes(precision, 1);
     if (precision == 1)
         hipLaunchKernelGGL(k_density<double>, dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p);
@@ -2472,7 +2497,7 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
                   int precision, int style, int mode, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, bl.perbead, style};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, bl.perbead, style, bl.phase};
     const size_t lds = bl.lds_bytes(precision, style);
     if (precision == 1) {
         if (style == 2)

@@ -103,6 +103,13 @@ class Context:
         self.check(self.lib.gdpd_kernel_timing_get(self.h, C.byref(ms), C.byref(n), C.byref(tot)))
         return ms.value, n.value, tot.value
 
+    def kernel_phases(self):
+        """Mean microseconds per wave and timed force launch: staging, pair
+        work, closing-barrier wait, write-back."""
+        out = np.zeros(4)
+        self.check(self.lib.gdpd_kernel_phases(self.h, _ptr(out, C.c_double)))
+        return dict(zip(["stage_us", "pairs_us", "barrier_us", "writeback_us"], out.tolist()))
+
 
 def make_desc(n_beads, n_types, lo, hi, dt, global_cut=1.0, skin=0.3, max_rho=6.0,
               check_every=1, log_every=100, precision=L.PREC_F32):

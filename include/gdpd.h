@@ -271,6 +271,14 @@ int gdpd_set_capacity(gdpd_ctx *ctx, int sys, int smax, int omax);
 int gdpd_kernel_timing(gdpd_ctx *ctx, int on);
 int gdpd_kernel_timing_get(gdpd_ctx *ctx, double *force_ms, int64_t *force_launches,
                            double *total_ms);
+/* Phases of the timed force launches (no counterpart in the reference; a
+   measurement aid like gdpd_kernel_timing).  While kernel timing is on, each
+   wave of the pair-force kernel reads the device clock at its phase
+   boundaries; us4 receives the mean microseconds per wave and launch of:
+   staging the brick into LDS, the wave's candidate search and pair
+   evaluation, its wait at the closing barrier, and the write-back of forces
+   and thermo partial sums.  Reset by gdpd_kernel_timing. */
+int gdpd_kernel_phases(gdpd_ctx *ctx, double *us4);
 /* waits for the context's stream, then settles held systems (see gdpd_advance) */
 int gdpd_sync(gdpd_ctx *ctx);
 /* Admission window, for serving loops that admit new systems while others run.

@@ -284,3 +284,31 @@ def test_admission_window(ctx):
         assert np.array_equal(p, r[0]) and np.array_equal(v, r[1])
         assert np.array_equal(s.thermo_rows(), r[2])
         s.destroy()
+
+
+def test_kernel_phases_measure_timed_force_launches(ctx):
+    """With kernel timing on, every wave of the force kernel adds its phase
+    durations; the phases are positive, the timed results equal the untimed
+    ones bit for bit (timing adds clock reads only), and a reset clears them."""
+    cfg = configs.scaled_dpd(6000, seed=21)
+    ref = g.system_from_config(ctx, cfg, precision=g.PREC_F32)
+    ref.run(6)
+    want = ref.download()
+    ref.destroy()
+    s = g.system_from_config(ctx, cfg, precision=g.PREC_F32)
+    ctx.kernel_timing(True)
+    s.run(6)
+    ms, n, _ = ctx.kernel_timing_get()
+    ph = ctx.kernel_phases()
+    ctx.kernel_timing(False)
+    got = s.download()
+    assert n == 7 and ms > 0
+    assert ph["stage_us"] > 0 and ph["pairs_us"] > 0 and ph["writeback_us"] > 0 and ph["barrier_us"] >= 0
+    # a wave's phases fit inside the summed launch time
+    assert sum(ph.values()) * 1e-3 < ms
+    for k in ("pos", "vel", "force"):
+        assert np.array_equal(got[k], want[k])
+    ctx.kernel_timing(True)
+    assert all(v == 0 for v in ctx.kernel_phases().values())
+    ctx.kernel_timing(False)
+    s.destroy()
