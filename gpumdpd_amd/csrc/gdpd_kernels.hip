@@ -605,10 +605,21 @@ __device__ __forceinline__ int real_to_id(double v) { return (int)__double_as_lo
 __device__ __forceinline__ int cell_runs(const BrickGeom &g, const int *c_start, unsigned *runs, int rx, int ry,
                                          int rz)
 {
-    const int own = (rx * g.RY + ry) * g.RZ + rz;
-    const int dend = c_start[own + 2];
+    // Every cell start the list needs is read before the first entry is
+    // written, so the 36 reads go out together instead of one LDS round trip
+    // per run.  cs[r][k] = c_start[cb + k] for neighbour column r, with cb its
+    // cell below the owned cell's layer; the own column is r = 4.
+    int cs[9][4];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
+        const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cs[r][k] = c_start[cb + k];
+    }
+    const int dend = cs[4][3];
     runs[0] = (unsigned)dend;
-    int n = 1, total = dend - c_start[own];
+    int n = 1, total = dend - cs[4][1];
     const bool zlo = rz == 1, zhi = rz == g.ext[2], thin = g.ext[2] == 1;
     auto put = [&](int js, int je) {
         if (je > js) {
@@ -616,20 +627,20 @@ __device__ __forceinline__ int cell_runs(const BrickGeom &g, const int *c_start,
             total += je - js;
         }
     };
+#pragma unroll
     for (int r = 0; r < 9; ++r) {
         const int cx = rx + r / 3 - 1, cy = ry + r % 3 - 1;
-        const int cb = (cx * g.RY + cy) * g.RZ + rz - 1;
-        const int A = c_start[cb], D = c_start[cb + 3];
+        const int A = cs[r][0], D = cs[r][3];
         if (r < 4) {
             const bool xyhalo = cx == 0 || cy == 0 || cy == g.RY - 1;
             if (xyhalo) {
                 put(A, D);
             } else {
-                if (zlo) put(A, c_start[cb + 1]);
-                if (zhi && (thin || !zlo)) put(c_start[cb + 2], D);
+                if (zlo) put(A, cs[r][1]);
+                if (zhi && (thin || !zlo)) put(cs[r][2], D);
             }
         } else if (r == 4) {
-            if (zlo) put(A, c_start[cb + 1]);
+            if (zlo) put(A, cs[r][1]);
         } else {
             put(A, D);
         }
@@ -732,18 +743,21 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     __syncthreads();
     const int nown = g.ext[0] * g.ext[1] * g.ext[2];
     {
-        // region-cell starts, and the run-list base of every owned cell (after
-        // the shared empty entry runs[0]), in one pair of scans
+        // region-cell starts (cells 4t..4t+3 per thread, in order), and the
+        // run-list base of every owned cell (cells t, t + 256, ... per thread:
+        // each cell needs its own range after the shared empty entry runs[0],
+        // in any order), in one pair of scans
         int v[4], c[4], loc = 0, cloc = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int rc = 4 * t + k;
             v[k] = (rc < g.nreg) ? L.c_start[rc + 1] : 0;
             loc += v[k];
+            const int oc = t + k * kBlock;
             c[k] = 0;
-            if (rc < nown) {
+            if (oc < nown) {
                 int rx, ry, rz;
-                split3(rc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
+                split3(oc, g.ext[1], g.ext[2], inv_ez, inv_eyz, rx, ry, rz);
                 c[k] = cell_run_cap(g.ext[0], g.ext[1], g.ext[2], rx + 1, ry + 1, rz + 1);
             }
             cloc += c[k];
@@ -755,8 +769,9 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         for (int k = 0; k < 4; ++k) {
             const int rc = 4 * t + k;
             if (rc < g.nreg) L.c_start[rc] = run;
-            if (rc < nown) L.rbase[rc] = crun;
             run += v[k];
+            const int oc = t + k * kBlock;
+            if (oc < nown) L.rbase[oc] = crun;
             crun += c[k];
         }
         if (t == 0) {
