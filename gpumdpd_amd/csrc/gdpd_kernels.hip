@@ -804,24 +804,50 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     bool own[4];
     double xs[4], ys[4], zs[4], vxs[4], vys[4], vzs[4];
     int ids[4], tys[4];
+    // A slot u whose staged indices all lie past the staged count for the
+    // whole wave is skipped (a wave-uniform branch), so no wasted permutation
+    // and state loads are issued for it: at rho = 3 a 4x4x5 brick stages about
+    // 760 beads, and no wave has a bead in its fourth slot.
+    const int lane = t & 63;
+    auto wave_has = [&](int base, int u) {
+        return __builtin_amdgcn_readfirstlane(base - lane) + u * kBlock < Sc;
+    };
+    // Stage by stage over the four slots, so each stage's LDS reads go out
+    // together: three LDS round trips before the permutation loads, not twelve.
     auto lookup = [&](int base) {
+        int lo[4], gs[4], cs0[4];
+        bool in[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = base + u * kBlock;
-            const bool in = k < Sc;
-            const int lo = in ? (int)L.cellof[k] : 0;
-            rcs[u] = lo;
-            const int gp = in ? L.gstart[lo] + (k - L.c_start[lo]) : -1;
-            const int ci = L.cinfo[lo];
-            const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
-            cis[u] = ci;
-            own[u] = rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
-            sl[u] = gp >= 0 ? perm[gp] : 0;
+            in[u] = k < Sc;
+            lo[u] = in[u] ? (int)L.cellof[k] : 0;
         }
-    };
-    auto fetch = [&]() {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
+            gs[u] = L.gstart[lo[u]];
+            cs0[u] = L.c_start[lo[u]];
+            cis[u] = L.cinfo[lo[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = base + u * kBlock;
+            rcs[u] = lo[u];
+            const int ci = cis[u];
+            const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
+            own[u] = in[u] && rx >= 1 && rx <= g.ext[0] && ry >= 1 && ry <= g.ext[1] && rz >= 1 && rz <= g.ext[2];
+            sl[u] = 0;
+            if (!wave_has(base, u)) continue;
+            if (in[u]) sl[u] = perm[gs[u] + (k - cs0[u])];
+        }
+    };
+    auto fetch = [&](int base) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            xs[u] = ys[u] = zs[u] = 0.0;
+            tys[u] = ids[u] = 0;
+            vxs[u] = vys[u] = vzs[u] = 0.0;
+            if (!wave_has(base, u)) continue;
             sl[u] = ((unsigned)sl[u] < (unsigned)N) ? sl[u] : 0;
             const int sv = sl[u];
             xs[u] = X[sv];
@@ -829,7 +855,6 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             zs[u] = X[2 * N + sv];
             tys[u] = gtype[sv];
             ids[u] = need_vel ? gid[sv] : 0;
-            vxs[u] = vys[u] = vzs[u] = 0.0;
             if (need_vel && own[u]) {
                 vxs[u] = V[sv];
                 vys[u] = V[N + sv];
@@ -858,7 +883,7 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
         if (t < kSigClasses) L.sigcnt[t] = inc - v;
         if (t == kSigClasses - 1) L.sigcnt[kSigClasses] = inc;
     }
-    fetch();
+    fetch(t);
     lds_barrier();
     for (int oc = t; oc < nown; oc += kBlock) {
         const int cp = L.cellpre[oc];
@@ -871,7 +896,7 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     for (int base = t; base < S; base += 4 * kBlock) {
         if (base != t) {
             lookup(base);
-            fetch();
+            fetch(base);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1265,7 +1290,8 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     if (halted(R)) return;
-    const int cur = R.state->cur;
+    // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
+    const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
     const BrickGeom g = brick_geom(R, it.local);
     const int O = stage_brick<real>(R, g, L, bp, cur, false, false);
     if (O < 0) {
@@ -1298,7 +1324,8 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     if (halted(R)) return;
     unsigned long long tp[5] = {0, 0, 0, 0, 0};
     if (bp.phase) tp[0] = wall_clock64();
-    const int cur = R.state->cur;
+    // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
+    const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
     const long long step = R.state->step;
     const bool log = (mode & kModeLogForce) ||
                      ((mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0);
