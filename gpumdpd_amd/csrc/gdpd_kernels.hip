@@ -1050,39 +1050,55 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             if (jown) lds_add(&L.facc[oj], w);
             continue;
         }
-        // force/pair_dpd.cu: pairs inside rc with r >= 1e-10
-        if (r2 > C.rc2 || r2 < (real)1.0e-20) continue;
-        // the HBM loads of a halo partner's velocity are issued first and used
-        // after the hash, so their latency overlaps it; lanes with an owned
-        // partner load slot 0 (one shared address) and ignore the value
+        // Every load of the pair is issued before the hash, which needs only
+        // the two ids, and the scheduler is kept from moving them below it
+        // (sched_barrier): its ~140 instructions then cover the latency of the
+        // coefficient, velocity and HBM loads.  The cutoff test masks the sums
+        // instead of branching (a lane whose pair fails it idles through the
+        // round either way).  The HBM loads of a halo partner's velocity are
+        // used after the hash; lanes with an owned partner load slot 0 (one
+        // shared address) and ignore the value.
         const int sv = (!jown && (unsigned)sj < (unsigned)N) ? sj : 0;
         const double gvx = V[sv], gvy = V[N + sv], gvz = V[2 * N + sv];
-        real r, rinv;
-        root_and_inverse(r2, r, rinv);
-        const real wc = (real)1 - ratio<real>(r, C.rc, C.rcinv);
-        const real wr = wc;
+        const real4 vi = L.vown[oi], vo = L.vown[jown ? oj : oi];
+        const real cA = C.A, cgamma = C.gamma, csig = C.sig, crc2 = C.rc2;
+        const real crc = C.rc, crcinv = C.rcinv;
+        real cB = (real)0, crd = (real)0, crdinv = (real)0, rhoi = (real)0, rhoj = (real)0;
+        if (STYLE == 2) {
+            cB = C.B;
+            crd = C.rd;
+            crdinv = C.rdinv;
+            rhoi = L.rh[li];
+            rhoj = L.rh[lj];
+        }
+        __builtin_amdgcn_sched_barrier(0);
         const int idi = word_id(wi), idj = word_id(wj);
         const uint32_t t1 = (uint32_t)min(idi, idj), t2 = (uint32_t)max(idi, idj);
-        const real rnd = saru_uniform<real>(saru_bits(t1, t2, seed_step));
-        const real4 vi = L.vown[oi], vo = L.vown[jown ? oj : oi];
+        const uint32_t hbits = saru_bits(t1, t2, seed_step);
+        // force/pair_dpd.cu: pairs inside rc with r >= 1e-10
+        const bool in_rc = !(r2 > crc2 || r2 < (real)1.0e-20);
+        real r, rinv;
+        root_and_inverse(r2, r, rinv);
+        const real wc = (real)1 - ratio<real>(r, crc, crcinv);
+        const real wr = wc;
+        const real rnd = saru_uniform<real>(hbits);
         const real vjx = jown ? vo.x : (real)gvx, vjy = jown ? vo.y : (real)gvy, vjz = jown ? vo.z : (real)gvz;
         const real dot = dx * (vi.x - vjx) + dy * (vi.y - vjy) + dz * (vi.z - vjz);
         real wd = (real)0, fpair;
-        real rhoi = (real)0, rhoj = (real)0;
         if (STYLE == 2) {
-            rhoi = L.rh[li];
-            rhoj = L.rh[lj];
-            wd = (real)1 - ratio<real>(r, C.rd, C.rdinv);
+            wd = (real)1 - ratio<real>(r, crd, crdinv);
             wd = wd > (real)0 ? wd : (real)0;
-            fpair = C.A * wc + C.B * (rhoi + rhoj) * wd;
+            fpair = cA * wc + cB * (rhoi + rhoj) * wd;
         } else {
-            fpair = C.A * wc;
+            fpair = cA * wc;
         }
-        fpair -= C.gamma * wr * wr * dot * rinv;
-        fpair += C.sig * wr * rnd;
+        fpair -= cgamma * wr * wr * dot * rinv;
+        fpair += csig * wr * rnd;
         fpair *= rinv;
-        // the force components times the fixed-point scale (exact), straight to fixed point
-        const real fps = fpair * (real)FixScale<real>::s;
+        // the force components times the fixed-point scale (exact), straight to
+        // fixed point; a pair outside the cutoff adds exact zeros, so the sums
+        // need no branch (a branch here would move the loads above behind it)
+        const real fps = in_rc ? fpair * (real)FixScale<real>::s : (real)0;
         const long long qx = fix_scaled<real>(dx * fps), qy = fix_scaled<real>(dy * fps),
                         qz = fix_scaled<real>(dz * fps);
         long long *fi = lds_elem(L.facc, (unsigned)oi, 3u * sizeof(long long));
@@ -1095,7 +1111,7 @@ __device__ __forceinline__ void eval_queue(const RepDev &R, const BrickLds<real>
             lds_sub(fj + 1, qy);
             lds_sub(fj + 2, qz);
         }
-        if (log) {
+        if (log && in_rc) {
             // per-bead shares as in force/pair_dpd.cu:366-398 and pair_mdpd.cu:479-516
             real pe_i, pe_j;
             if (STYLE == 2) {
