@@ -1305,11 +1305,14 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
     const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
-    if (halted(R)) return;
+    // the halt flag is tested after staging, which only reads, so the brick's
+    // first loads do not wait for the state block
+    const bool held = halted(R);
     // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
     const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
     const BrickGeom g = brick_geom(R, it.local);
     const int O = stage_brick<real>(R, g, L, bp, cur, false, false);
+    if (held) return;
     if (O < 0) {
         if (threadIdx.x == 0) atomicOr(&R.state->flags, kFlagStageOverflow);
         return;
@@ -1337,9 +1340,11 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     const int item = xcd_item(nitems);
     const WorkItem it = items[item];
     const RepDev &R = reps[it.rep];
-    if (halted(R)) return;
     unsigned long long tp[5] = {0, 0, 0, 0, 0};
     if (bp.phase) tp[0] = wall_clock64();
+    // the halt flag is tested after staging, which only reads, so the brick's
+    // first loads do not wait for the state block
+    const bool held = halted(R);
     // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
     const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
     const long long step = R.state->step;
@@ -1347,6 +1352,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
                      ((mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0);
     const BrickGeom g = brick_geom(R, it.local);
     const int O = stage_brick<real>(R, g, L, bp, cur, true, STYLE == 2);
+    if (held) return;
     if (O < 0) {
         if (threadIdx.x == 0) atomicOr(&R.state->flags, kFlagStageOverflow);
         return;
@@ -1454,7 +1460,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     if (halted(R)) return;  // the force of this step is incomplete: hold the state
-    const int cur = R.state->cur;
+    // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
+    const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
     const long long step = R.state->step;
     const bool log = (mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0;
     const int N = R.N;
