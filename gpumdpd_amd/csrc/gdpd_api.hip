@@ -437,6 +437,22 @@ int rebin(gdpd_ctx *ctx, Tables &T)
     return check_launch(ctx);
 }
 
+// Physical re-sort of every system of T into cell order.  The device swaps the
+// id, type and image arrays with their spares (k_resort_swap); the host copies
+// of the descriptors swap with them, so later descriptor uploads agree.
+int resort_tables(gdpd_ctx *ctx, Tables &T)
+{
+    launch_resort(ctx->d_reps, T.beads, T.n_beads, T.rep_list, T.n_reps, ctx->stream);
+    for (int id : T.key) {
+        Sys *s = (id >= 0 && id < (int)ctx->sys.size()) ? ctx->sys[id] : nullptr;
+        if (!s) continue;
+        std::swap(s->h.id, s->h.id_alt);
+        std::swap(s->h.type, s->h.type_alt);
+        std::swap(s->h.img, s->h.img_alt);
+    }
+    return check_launch(ctx);
+}
+
 // 0: no system of T has bonds or angles, 1: bonds only, 2: angles too
 int topo_of(gdpd_ctx *ctx, const Tables &T)
 {
@@ -718,8 +734,7 @@ int prepare(gdpd_ctx *ctx, Sys *s)
 
     // initial binning and physical sort into cell order
     if ((rc = rebin(ctx, s->tab))) return rc;
-    launch_resort(ctx->d_reps, s->tab.beads, s->tab.n_beads, ctx->stream);
-    if ((rc = check_launch(ctx))) return rc;
+    if ((rc = resort_tables(ctx, s->tab))) return rc;
     std::vector<int> cs_host((size_t)h.ncell + 1);
     HIPCHK(hipMemcpyAsync(cs_host.data(), h.cell_start, sizeof(int) * cs_host.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -761,9 +776,8 @@ int maybe_resort(gdpd_ctx *ctx, Sys *s, Tables &T, int steps)
     s->steps_since_resort += steps;
     const int every = s->resort_every > 0 ? s->resort_every : kDefaultResortEvery;
     if (s->steps_since_resort >= every) {
-        launch_resort(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
         s->steps_since_resort = 0;
-        return check_launch(ctx);
+        return resort_tables(ctx, T);
     }
     return GDPD_OK;
 }
@@ -825,8 +839,7 @@ int advance_tables(gdpd_ctx *ctx, Tables &T, Graph &G, int nsteps, int graph_ste
             due = due || s->steps_since_resort >= every;
         }
         if (due) {
-            launch_resort(ctx->d_reps, T.beads, T.n_beads, ctx->stream);
-            if ((rc = check_launch(ctx))) return rc;
+            if ((rc = resort_tables(ctx, T))) return rc;
             for (Sys *s : members) s->steps_since_resort = 0;
         }
         done += chunk;
@@ -1336,7 +1349,9 @@ int gdpd_sys_create(gdpd_ctx *ctx, const gdpd_system_desc *d, int *sys_out)
     ALLOC(h.angle_start, (size_t)N + 1);
     ALLOC(h.vir, 6 * (size_t)N);
     ALLOC(h.rho, N);
-    ALLOC(h.iscratch, 4 * (size_t)N);
+    ALLOC(h.id_alt, N);
+    ALLOC(h.type_alt, N);
+    ALLOC(h.img_alt, N);
     ALLOC(h.state, 1);
     ALLOC(h.step_ts, kStepTsCap);
     ALLOC(h.racc, (size_t)kRecSlots * kAccPer * 2);
@@ -1459,8 +1474,7 @@ int gdpd_upload(gdpd_ctx *ctx, int sid, const int *id, const int *type, const do
         // capacities for it (the brick shape is kept while it fits)
         launch_fill_i32(h.cell_count, 0, h.ncell, ctx->stream);
         if ((rc = rebin(ctx, s->tab))) return rc;
-        launch_resort(ctx->d_reps, s->tab.beads, s->tab.n_beads, ctx->stream);
-        if ((rc = check_launch(ctx))) return rc;
+        if ((rc = resort_tables(ctx, s->tab))) return rc;
         std::vector<int> cs((size_t)h.ncell + 1);
         HIPCHK(hipMemcpyAsync(cs.data(), h.cell_start, sizeof(int) * cs.size(), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -136,8 +136,11 @@ struct RepDev {
     double *x0uw;              // [3N] by id-1: msd reference (unwrapped) positions
     RepState *state;
     unsigned long long *step_ts;  // [kStepTsCap]: device clock when step s completed, at s % kStepTsCap
-    // scratch for re-sorting
-    int *iscratch;             // 4N ints: id, type, then the 64-bit image words
+    // spare id, type and image arrays: a physical re-sort writes the
+    // permuted words into them and swaps them in (k_resort, k_resort_swap)
+    int *id_alt;
+    unsigned char *type_alt;
+    long long *img_alt;
 };
 
 // Run-list entries an owned cell (rx, ry, rz), 1-based in a brick whose owned

@@ -24,7 +24,9 @@ void launch_cellsort(const RepDev *reps, const WorkItem *items, int nitems, hipS
 // scan over cell counts: items of 4*kBlock cells; chunk_off[r] = first chunk of rep r
 void launch_scan(const RepDev *reps, const WorkItem *items, int nitems, const int *rep_chunk_off,
                  const int *rep_list, int nreps, int *chunk_sums, hipStream_t s);
-void launch_resort(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s);
+// physical re-sort of the listed replicas; the host then swaps its copies of
+// the id / id_alt, type / type_alt and img / img_alt pointers (resort_tables)
+void launch_resort(RepDev *reps, const WorkItem *items, int nitems, const int *rep_list, int nreps, hipStream_t s);
 // bricks (items: rep, brick)
 void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
                     int precision, hipStream_t s);

@@ -315,44 +315,51 @@ __global__ __launch_bounds__(kBlock) void k_cellsort(const RepDev *__restrict__ 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_resort_gather(const RepDev *__restrict__ reps,
-                                                          const WorkItem *__restrict__ items,
-                                                          int nitems)
+// Physical re-sort into cell order, one pass: slot d takes the bead at sorted
+// position d (perm[d]).  Positions and velocities go to the other state
+// buffer, the id, type and image words to the spare arrays, and perm becomes
+// the identity; k_resort_swap then flips the buffer parity and swaps the spare
+// arrays in.  One pass over the bead arrays instead of a gather into scratch
+// and a copy back.
+__global__ __launch_bounds__(kBlock) void k_resort(const RepDev *__restrict__ reps,
+                                                   const WorkItem *__restrict__ items, int nitems)
 {
     const WorkItem it = items[xcd_item(nitems)];
     const RepDev &R = reps[it.rep];
     const int d = it.local * kBlock + threadIdx.x;
     if (d >= R.N) return;
-    const int N = R.N, cur = R.state->cur;
-    const int s = R.perm[d];
-    const double *Xi = R.x[cur], *Vi = R.v[cur];
-    double *Xo = R.x[1 - cur], *Vo = R.v[1 - cur];
+    const int N = R.N;
+    const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
+    const int s = gmem(R.perm)[d];
+    const auto *Xi = gmem(R.x[cur]), *Vi = gmem(R.v[cur]);
+    auto *Xo = gmemw(R.x[1 - cur]), *Vo = gmemw(R.v[1 - cur]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         Xo[k * N + d] = Xi[k * N + s];
         Vo[k * N + d] = Vi[k * N + s];
     }
-    R.iscratch[d] = R.id[s];
-    R.iscratch[N + d] = R.type[s];
-    ((long long *)(R.iscratch + 2 * (size_t)N))[d] = R.img[s];
+    const int id = gmem(R.id)[s];
+    gmemw(R.id_alt)[d] = id;
+    gmemw(R.type_alt)[d] = gmem(R.type)[s];
+    gmemw(R.img_alt)[d] = gmem(R.img)[s];
+    gmemw(R.perm)[d] = d;
+    if ((unsigned)(id - 1) < (unsigned)N) R.slot_of_id[id - 1] = d;
 }
 
-__global__ __launch_bounds__(kBlock) void k_resort_finish(const RepDev *__restrict__ reps,
-                                                          const WorkItem *__restrict__ items,
-                                                          int nitems)
+// one thread per listed replica: the re-sorted buffers become the current ones
+__global__ void k_resort_swap(RepDev *reps, const int *__restrict__ rep_list)
 {
-    const WorkItem it = items[xcd_item(nitems)];
-    const RepDev &R = reps[it.rep];
-    const int d = it.local * kBlock + threadIdx.x;
-    if (it.local == 0 && threadIdx.x == 0) R.state->cur ^= 1;
-    if (d >= R.N) return;
-    const int N = R.N;
-    const int id = R.iscratch[d];
-    R.id[d] = id;
-    R.type[d] = (unsigned char)R.iscratch[N + d];
-    R.img[d] = ((const long long *)(R.iscratch + 2 * (size_t)N))[d];
-    R.perm[d] = d;
-    if ((unsigned)(id - 1) < (unsigned)N) R.slot_of_id[id - 1] = d;
+    RepDev &R = reps[rep_list[blockIdx.x]];
+    R.state->cur ^= 1;
+    int *id = R.id;
+    R.id = R.id_alt;
+    R.id_alt = id;
+    unsigned char *ty = R.type;
+    R.type = R.type_alt;
+    R.type_alt = ty;
+    long long *im = R.img;
+    R.img = R.img_alt;
+    R.img_alt = im;
 }
 
 // ---------------------------------------------------------------------------
@@ -2541,11 +2548,11 @@ void launch_scan(const RepDev *reps, const WorkItem *items, int nitems, const in
     hipLaunchKernelGGL(k_scan_down, dim3(nitems), dim3(kBlock), 0, s, reps, items,
                        (const int *)chunk_sums);
 }
-void launch_resort(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s)
+void launch_resort(RepDev *reps, const WorkItem *items, int nitems, const int *rep_list, int nreps, hipStream_t s)
 {
     if (!nitems) return;
-    hipLaunchKernelGGL(k_resort_gather, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems);
-    hipLaunchKernelGGL(k_resort_finish, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems);
+    hipLaunchKernelGGL(k_resort, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems);
+    hipLaunchKernelGGL(k_resort_swap, dim3(nreps), dim3(1), 0, s, reps, rep_list);
 }
 void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const BrickLaunch &bl,
                     int precision, hipStream_t s)
