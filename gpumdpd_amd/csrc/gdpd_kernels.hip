@@ -53,6 +53,26 @@ template <typename T> __device__ __forceinline__ __attribute__((address_space(1)
 // The flag is per replica, so the test is uniform across a workgroup.
 __device__ __forceinline__ bool halted(const RepDev &R) { return (R.state->flags & kFlagHalt) != 0; }
 
+// A replica's step counter as a wave-uniform value, so what derives from it
+// (the log decision, the Saru step key) stays in scalar registers.
+__device__ __forceinline__ long long uniform_step(const RepDev &R)
+{
+    const long long v = R.state->step;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// step % every == 0 for a log interval every > 0.  Steps below 2^31 (every
+// run in practice) take a 32-bit remainder, a few instructions; the general
+// 64-bit remainder expands to about 85 vector instructions per wave.
+__device__ __forceinline__ bool log_due(long long step, int every)
+{
+    if (every <= 0) return false;
+    if ((unsigned long long)step < 0x80000000ull) return (unsigned)step % (unsigned)every == 0u;
+    return step % every == 0;
+}
+
 
 __device__ __forceinline__ int cell_axis(double x, double lo, double inv_cs, int nc)
 {
@@ -535,6 +555,7 @@ __device__ __forceinline__ BrickLds<real> lds_bind(char *smem, const LdsOffsets 
 
 struct BrickGeom {
     int c0[3], ext[3], RX, RY, RZ, nreg;
+    bool wraps;  // some region cell lies across a periodic face (image shift nonzero)
     double org[3];
 };
 
@@ -553,13 +574,21 @@ __device__ __forceinline__ BrickGeom brick_geom(const RepDev &R, int b)
     g.RY = g.ext[1] + 2;
     g.RZ = g.ext[2] + 2;
     g.nreg = g.RX * g.RY * g.RZ;
+    g.wraps = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g.wraps = g.wraps || g.c0[d] == 0 || g.c0[d] + g.ext[d] >= R.nc[d];
     return g;
 }
 
-// n / d for 0 <= n < 2^22 and 1 <= d <= 2^10 from a float reciprocal of d:
-// (n + 0.5) / d lies at least 1/(2d) from an integer, far more than the float
-// error of the product, so truncation gives the exact quotient
+// n / d for 0 <= n < 2^16 and 1 <= d <= 2^10 from a float reciprocal of d
+// within 2 ulp: (n + 0.5) / d lies at least 1/(2d) from an integer, far more
+// than the float error of the product, so truncation gives the exact quotient
+// (checked for every such n and d with the reciprocal 2 ulp either side)
 __device__ __forceinline__ int small_div(int n, float dinv) { return (int)(((float)n + 0.5f) * dinv); }
+
+// 1/d for small_div: the hardware reciprocal (1 ulp), one instruction
+// instead of the expansion of a correctly rounded division
+__device__ __forceinline__ float small_rcp(int d) { return __builtin_amdgcn_rcpf((float)d); }
 
 // region cell rc -> (rx, ry, rz) with rc = (rx * RY + ry) * RZ + rz
 __device__ __forceinline__ void split3(int rc, int RY, int RZ, float inv_rz, float inv_ryz, int &rx, int &ry, int &rz)
@@ -726,8 +755,8 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             cw[u] = k < ncw ? cg[k] : 0;
         }
     }
-    const float inv_rz = 1.0f / (float)g.RZ, inv_ryz = 1.0f / (float)(g.RY * g.RZ);
-    const float inv_ez = 1.0f / (float)g.ext[2], inv_eyz = 1.0f / (float)(g.ext[1] * g.ext[2]);
+    const float inv_rz = small_rcp(g.RZ), inv_ryz = small_rcp(g.RY * g.RZ);
+    const float inv_ez = small_rcp(g.ext[2]), inv_eyz = small_rcp(g.ext[1] * g.ext[2]);
     for (int rc = t; rc < g.nreg; rc += kBlock) {
         int rx, ry, rz;
         split3(rc, g.RY, g.RZ, inv_rz, inv_ryz, rx, ry, rz);
@@ -912,13 +941,18 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             const int rc2 = rcs[u];
             const int ci = cis[u];
             const int rx = ci & 255, ry = (ci >> 8) & 255, rz = (ci >> 16) & 255;
-            const double shx = (double)(((ci >> 24) & 3) - 1) * R.L[0];
-            const double shy = (double)(((ci >> 26) & 3) - 1) * R.L[1];
-            const double shz = (double)(((ci >> 28) & 3) - 1) * R.L[2];
+            // periodic image, then brick-relative: (x + shift) - org, with the
+            // shift add left out where every shift is zero (x + 0 is x)
+            double px = xs[u], py = ys[u], pz = zs[u];
+            if (g.wraps) {
+                px += (double)(((ci >> 24) & 3) - 1) * R.L[0];
+                py += (double)(((ci >> 26) & 3) - 1) * R.L[1];
+                pz += (double)(((ci >> 28) & 3) - 1) * R.L[2];
+            }
             real4 p;
-            p.x = (real)((xs[u] + shx) - g.org[0]);
-            p.y = (real)((ys[u] + shy) - g.org[1]);
-            p.z = (real)((zs[u] + shz) - g.org[2]);
+            p.x = (real)(px - g.org[0]);
+            p.y = (real)(py - g.org[1]);
+            p.z = (real)(pz - g.org[2]);
             p.w = id_to_real(stage_word(ids[u], tys[u], !own[u]), (real)0);
             L.pos[k] = p;
             if (own[u]) {
@@ -1354,9 +1388,9 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     const bool held = halted(R);
     // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
     const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
-    const long long step = R.state->step;
+    const long long step = uniform_step(R);
     const bool log = (mode & kModeLogForce) ||
-                     ((mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0);
+                     ((mode & kModeLogAuto) && log_due(step, R.log_every));
     const BrickGeom g = brick_geom(R, it.local);
     const int O = stage_brick<real>(R, g, L, bp, cur, true, STYLE == 2);
     if (held) return;
@@ -1469,8 +1503,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(const RepDev *__restrict__
     if (halted(R)) return;  // the force of this step is incomplete: hold the state
     // uniform: the state pointers R.x[cur], R.v[cur] then come from scalar loads
     const int cur = __builtin_amdgcn_readfirstlane(R.state->cur);
-    const long long step = R.state->step;
-    const bool log = (mode & kModeLogAuto) && R.log_every > 0 && (step % R.log_every) == 0;
+    const long long step = uniform_step(R);
+    const bool log = (mode & kModeLogAuto) && log_due(step, R.log_every);
     const int N = R.N;
     const int s = it.local * kBlock + threadIdx.x;
     double acc[kThermoParts];
@@ -1650,8 +1684,8 @@ __global__ __launch_bounds__(kBlock) void k_step_end(const RepDev *__restrict__ 
     __shared__ long long itot[kThermoParts];
     const RepDev &R = reps[rep_list[blockIdx.x]];
     if (halted(R)) return;  // no record, no step count, no buffer flip
-    const long long step = R.state->step;
-    if (thermo && R.log_every > 0 && (step % R.log_every) == 0) {
+    const long long step = uniform_step(R);
+    if (thermo && log_due(step, R.log_every)) {
         // exact integer sums of the fixed-point partials
         long long acc[kThermoParts];
 #pragma unroll
