@@ -383,6 +383,14 @@ int build_tables(gdpd_ctx *ctx, const std::vector<int> &ids, Tables &T)
         bl_max(s->style == GDPD_STYLE_MDPD ? T.bl_mdpd : T.bl_dpd, b);
         T.precision = s->h.precision;
     }
+    // a launch whose bricks all belong to one replica, in order, skips the item reads
+    auto single = [](const std::vector<WorkItem> &v) {
+        for (size_t k = 0; k < v.size(); ++k)
+            if (v[k].rep != v[0].rep || v[k].local != (int)k) return -1;
+        return v.empty() ? -1 : v[0].rep;
+    };
+    T.bl_dpd.single_rep = single(bd);
+    T.bl_mdpd.single_rep = single(bm);
     int rc;
     if ((rc = upload_items(ctx, beads, &T.beads))) return rc;
     if ((rc = upload_items(ctx, cells, &T.cells))) return rc;

@@ -13,6 +13,7 @@ struct BrickLaunch {
     int qw = 192;  // pair-queue entries per wave: a flush leaves < 64, one candidate step adds <= 128
     int perbead = 0; // per-bead PE/virial sums (hook path log steps)
     unsigned *phase = nullptr;  // per item and wave: 4 phase-clock sums (kernel timing)
+    int single_rep = -1;        // >= 0: the launch's items are bricks 0.. of this replica, in order
     size_t lds_bytes(int precision, int style) const;
 };
 

@@ -406,7 +406,17 @@ struct BrickParams {
     // device-clock ticks of staging, pair work, the closing barrier and the
     // write-back; null when off
     unsigned *phase;
+    // >= 0: every item of the launch is brick `item` of this replica, in
+    // order, so the item need not be read (one dependent load less before the
+    // first cell-start loads); -1: read items[]
+    int single_rep;
 };
+
+__device__ __forceinline__ WorkItem brick_item(const WorkItem *items, int item, const BrickParams &bp)
+{
+    if (bp.single_rep >= 0) return WorkItem{bp.single_rep, item};
+    return items[item];
+}
 
 // lane 0 of each wave adds its phase durations (t[0..4] device clock) to the
 // work item's four counters; each counter has one writer, so plain adds
@@ -1344,7 +1354,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const RepDev *__restrict__ r
 {
     extern __shared__ __align__(16) char smem[];
     const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
-    const WorkItem it = items[xcd_item(nitems)];
+    const WorkItem it = brick_item(items, xcd_item(nitems), bp);
     const RepDev &R = reps[it.rep];
     // the halt flag is tested after staging, which only reads, so the brick's
     // first loads do not wait for the state block
@@ -1379,7 +1389,7 @@ __global__ __launch_bounds__(kBlock) void k_brick(const RepDev *__restrict__ rep
     extern __shared__ __align__(16) char smem[];
     const BrickLds<real> L = lds_bind<real>(smem, lds_offsets<real>(bp));
     const int item = xcd_item(nitems);
-    const WorkItem it = items[item];
+    const WorkItem it = brick_item(items, item, bp);
     const RepDev &R = reps[it.rep];
     unsigned long long tp[5] = {0, 0, 0, 0, 0};
     if (bp.phase) tp[0] = wall_clock64();
@@ -2592,7 +2602,7 @@ void launch_density(const RepDev *reps, const WorkItem *items, int nitems, const
                     int precision, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, 0, 1, nullptr};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, 0, 1, nullptr, bl.single_rep};
     const size_t lds = bl.lds_bytes(precision, 1);
     if (precision == 1)
         hipLaunchKernelGGL(k_density<double>, dim3(nitems), dim3(kBlock), lds, s, reps, items, nitems, p);
@@ -2603,7 +2613,7 @@ void launch_brick(const RepDev *reps, const WorkItem *items, int nitems, const B
                   int precision, int style, int mode, hipStream_t s)
 {
     if (!nitems) return;
-    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, bl.perbead, style, bl.phase};
+    BrickParams p{bl.rmax, bl.smax, bl.omax, bl.nrun, bl.ntypes, bl.qw, bl.perbead, style, bl.phase, bl.single_rep};
     const size_t lds = bl.lds_bytes(precision, style);
     if (precision == 1) {
         if (style == 2)
