@@ -931,11 +931,6 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
     }
     fetch(t);
     lds_barrier();
-    for (int oc = t; oc < nown; oc += kBlock) {
-        const int cp = L.cellpre[oc];
-        L.cellpre[oc] = (cp & 0xFFFF) + L.sigcnt[cp >> 16];
-    }
-    lds_barrier();
     const int O = L.sigcnt[kSigClasses];
     if (S > bp.smax || O > bp.omax) return -1;
 
@@ -967,7 +962,9 @@ __device__ __forceinline__ int stage_brick(const RepDev &R, const BrickGeom &g, 
             L.pos[k] = p;
             if (own[u]) {
                 const int oc = ((rx - 1) * g.ext[1] + (ry - 1)) * g.ext[2] + (rz - 1);
-                const int o = L.cellpre[oc] + (k - L.c_start[rc2]);
+                // the cell's offset inside its class, plus the class's offset
+                const int cp = L.cellpre[oc];
+                const int o = (cp & 0xFFFF) + L.sigcnt[cp >> 16] + (k - L.c_start[rc2]);
                 L.sinfo[k] = o;
                 L.oinfo[o] = owned_info(k, L.rbase[oc]);
                 real4 q;
