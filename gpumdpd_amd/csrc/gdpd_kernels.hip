@@ -298,17 +298,39 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const RepDev *__restrict__
     }
 }
 
+// Four bead chunks (work items) per workgroup, their loads issued together:
+// each chunk is a short chain of dependent accesses (key and rank, then the
+// cell start, then the store), which one chunk per thread leaves exposed.
+constexpr int kScatterItems = 4;
+
 __global__ __launch_bounds__(kBlock) void k_scatter(const RepDev *__restrict__ reps,
                                                     const WorkItem *__restrict__ items, int nitems)
 {
-    const WorkItem it = items[xcd_item(nitems)];
-    const RepDev &R = reps[it.rep];
-    const int s = it.local * kBlock + threadIdx.x;
-    if (s >= R.N || halted(R)) return;
-    const int key = gmem(R.key)[s];
-    const int pos = ((unsigned)key < (unsigned)R.ncell) ? gmem(R.cell_start)[key] + gmem(R.rank)[s] : -1;
-    if ((unsigned)pos < (unsigned)R.N) gmemw(R.perm)[pos] = s;
-    else atomicOr(&R.state->flags, kFlagScatter);
+    const int g0 = xcd_item((nitems + kScatterItems - 1) / kScatterItems) * kScatterItems;
+    int key[kScatterItems], rank[kScatterItems], s[kScatterItems];
+#pragma unroll
+    for (int u = 0; u < kScatterItems; ++u) {
+        key[u] = -1;
+        rank[u] = 0;
+        s[u] = -1;
+        if (g0 + u >= nitems) continue;
+        const WorkItem it = items[g0 + u];
+        const RepDev &R = reps[it.rep];
+        const int sl = it.local * kBlock + threadIdx.x;
+        if (sl >= R.N) continue;
+        s[u] = sl;
+        key[u] = gmem(R.key)[sl];
+        rank[u] = gmem(R.rank)[sl];
+    }
+#pragma unroll
+    for (int u = 0; u < kScatterItems; ++u) {
+        if (g0 + u >= nitems) continue;
+        const RepDev &R = reps[items[g0 + u].rep];
+        if (s[u] < 0 || halted(R)) continue;
+        const int pos = ((unsigned)key[u] < (unsigned)R.ncell) ? gmem(R.cell_start)[key[u]] + rank[u] : -1;
+        if ((unsigned)pos < (unsigned)R.N) gmemw(R.perm)[pos] = s[u];
+        else atomicOr(&R.state->flags, kFlagScatter);
+    }
 }
 
 // sort each cell's beads by id.  Used by the initial binning only, so a freshly
@@ -2573,7 +2595,9 @@ void launch_bin_keys(const RepDev *reps, const WorkItem *items, int nitems, hipS
 }
 void launch_scatter(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s)
 {
-    if (nitems) hipLaunchKernelGGL(k_scatter, dim3(nitems), dim3(kBlock), 0, s, reps, items, nitems);
+    if (nitems)
+        hipLaunchKernelGGL(k_scatter, dim3((nitems + kScatterItems - 1) / kScatterItems), dim3(kBlock), 0, s, reps,
+                           items, nitems);
 }
 void launch_cellsort(const RepDev *reps, const WorkItem *items, int nitems, hipStream_t s)
 {
